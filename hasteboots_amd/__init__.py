@@ -1,0 +1,21 @@
+"""MI355X-native TFHE gate-bootstrapping engine (HIP kernels for gfx950 behind a C ABI).
+
+The Python layer mirrors the reference `tfhe` crate's gate API; see tfhe.py.
+"""
+from .tfhe import (  # noqa: F401
+    BABYBEAR_BINARY_128_BITS_PARAMETERS,
+    GOLDILOCKS_BINARY_128_BITS_PARAMETERS,
+    ZAMA_BINARY_128_BITS_PARAMETERS,
+    SET_BABYBEAR,
+    SET_GOLDILOCKS,
+    SET_ZAMA,
+    Decryptor,
+    Encryptor,
+    Evaluator,
+    KeyGen,
+    LWECiphertext,
+    Parameters,
+    SecretKeyPack,
+    device_count,
+    last_kernel_ms,
+)

@@ -1,0 +1,575 @@
+// C ABI of the engine (include/hb.h): handle management, key upload to HBM in the
+// device layout, and launches of the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hb.h"
+#include "hb_field.hpp"
+#include "hb_kernels.hpp"
+#include "hb_keys.hpp"
+
+using namespace hb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(HB_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+// Per-device constant tables: twiddles for log N = 10, 11 in device word form.
+struct DeviceTables {
+  bool ready = false;
+  void* tw[2][2][2] = {};  // [field][logN-10][inverse]
+  uint64_t ninv[2][2] = {};  // device-form N^-1 for the microbench inverse
+};
+
+struct Engine {
+  std::mutex mu;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  DeviceTables tables;
+};
+
+Engine& engine() {
+  static Engine e;
+  return e;
+}
+
+uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
+  // scale: a canonical multiplier folded into the constant (e.g. N^-1).
+  if (p.is_goldilocks) return Gl::canon(Gl::mul(v, scale));
+  uint32_t x = (uint32_t)((v * scale) % p.q);
+  return Bb::to_mont(x);
+}
+
+int ensure_engine(int device) {
+  Engine& E = engine();
+  if (E.device >= 0) {
+    if (device != E.device) return fail(HB_ERR_ARG, "engine already bound to device %d", E.device);
+    HIP_TRY(hipSetDevice(E.device));
+    return HB_OK;
+  }
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(HB_ERR_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(HB_ERR_ARG, "device %d out of range (%d)", device, n);
+  HIP_TRY(hipSetDevice(device));
+  HIP_TRY(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&E.ev0));
+  HIP_TRY(hipEventCreate(&E.ev1));
+  // Twiddle tables.
+  for (int f = 0; f < 2; f++) {
+    ParamSet p;
+    param_set(f == 1 ? SET_GOLDILOCKS : SET_BABYBEAR, &p);
+    for (int lg = 10; lg <= 11; lg++) {
+      HostNtt ntt;
+      ntt.init(p, lg);
+      for (int inv = 0; inv < 2; inv++) {
+        const std::vector<uint64_t>& src = inv ? ntt.psi_inv_rev : ntt.psi_rev;
+        size_t bytes = src.size() * (f ? 8 : 4);
+        void* d = nullptr;
+        HIP_TRY(hipMalloc(&d, bytes));
+        if (f) {
+          HIP_TRY(hipMemcpy(d, src.data(), bytes, hipMemcpyHostToDevice));
+        } else {
+          std::vector<uint32_t> w(src.size());
+          for (size_t k = 0; k < src.size(); k++) w[k] = Bb::to_mont((uint32_t)src[k]);
+          HIP_TRY(hipMemcpy(d, w.data(), bytes, hipMemcpyHostToDevice));
+        }
+        E.tables.tw[f][lg - 10][inv] = d;
+      }
+      E.tables.ninv[f][lg - 10] = f ? ntt.n_inv : Bb::to_mont((uint32_t)ntt.n_inv);
+    }
+  }
+  E.tables.ready = true;
+  E.device = device;
+  return HB_OK;
+}
+
+uint32_t ilog2u(uint32_t x) {
+  uint32_t r = 0;
+  while ((1u << r) < x) r++;
+  return r;
+}
+
+}  // namespace
+
+struct hb_sk {
+  SecretKey sk;
+};
+
+struct hb_ek {
+  ParamSet p;
+  std::vector<uint64_t> bsk_host, ksk_host;  // canonical layout (kept for export)
+  void* bsk_dev = nullptr;
+  void* ksk_dev = nullptr;
+  void* lut_dev = nullptr;  // N words staging for hb_bootstrap
+  // staging for host-pointer calls and the accumulator scratch
+  std::mutex mu;
+  uint64_t* io_dev = nullptr;
+  size_t io_cap = 0;  // gates
+  void* acc_dev = nullptr;
+  size_t acc_cap = 0;  // gates
+};
+
+namespace {
+
+int upload_keys(hb_ek* ek) {
+  const ParamSet& p = ek->p;
+  const size_t wb = p.is_goldilocks ? 8 : 4;
+  const uint64_t nb = bsk_words(p), nk = ksk_words(p);
+  HostNtt ntt;
+  ntt.init(p, ilog2u(p.N));
+  const uint64_t ninv = ntt.n_inv;
+  auto convert = [&](const std::vector<uint64_t>& src, void** dst) -> int {
+    std::vector<uint8_t> buf(src.size() * wb);
+    if (p.is_goldilocks) {
+      uint64_t* w = reinterpret_cast<uint64_t*>(buf.data());
+      for (size_t k = 0; k < src.size(); k++) w[k] = to_device_word(p, src[k], ninv);
+    } else {
+      uint32_t* w = reinterpret_cast<uint32_t*>(buf.data());
+      for (size_t k = 0; k < src.size(); k++) w[k] = (uint32_t)to_device_word(p, src[k], ninv);
+    }
+    HIP_TRY(hipMalloc(dst, buf.size()));
+    HIP_TRY(hipMemcpy(*dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return HB_OK;
+  };
+  (void)nb;
+  (void)nk;
+  int rc = convert(ek->bsk_host, &ek->bsk_dev);
+  if (rc) return rc;
+  rc = convert(ek->ksk_host, &ek->ksk_dev);
+  if (rc) return rc;
+  HIP_TRY(hipMalloc(&ek->lut_dev, (size_t)p.N * 8));
+  return HB_OK;
+}
+
+int ensure_acc(hb_ek* ek, size_t count) {
+  if (count <= ek->acc_cap) return HB_OK;
+  if (ek->acc_dev) HIP_TRY(hipFree(ek->acc_dev));
+  ek->acc_dev = nullptr;
+  ek->acc_cap = 0;
+  const size_t wb = ek->p.is_goldilocks ? 8 : 4;
+  HIP_TRY(hipMalloc(&ek->acc_dev, count * 2 * ek->p.N * wb));
+  ek->acc_cap = count;
+  return HB_OK;
+}
+
+int ensure_io(hb_ek* ek, size_t count) {
+  if (count <= ek->io_cap) return HB_OK;
+  if (ek->io_dev) HIP_TRY(hipFree(ek->io_dev));
+  ek->io_dev = nullptr;
+  ek->io_cap = 0;
+  HIP_TRY(hipMalloc((void**)&ek->io_dev, count * 4 * (ek->p.n + 1) * sizeof(uint64_t)));
+  ek->io_cap = count;
+  return HB_OK;
+}
+
+void lut_params(const ParamSet& p, int gate, BootArgs& a) {
+  const uint32_t N = p.N;
+  const uint64_t q8 = p.q >> 3, nq8 = p.q - q8;
+  uint32_t mid;
+  uint64_t lo, hi;
+  switch (gate) {
+    case GATE_NAND: mid = (N >> 1) + (N >> 2); lo = q8; hi = nq8; break;
+    case GATE_AND:
+    case GATE_MAJORITY: mid = (N >> 1) + (N >> 2); lo = nq8; hi = q8; break;
+    case GATE_OR: mid = N >> 2; lo = nq8; hi = q8; break;
+    case GATE_NOR: mid = N >> 2; lo = q8; hi = nq8; break;
+    case GATE_XOR: mid = N >> 1; lo = nq8; hi = q8; break;
+    case GATE_XNOR: mid = N >> 1; lo = q8; hi = nq8; break;
+    default: mid = N; lo = hi = 0;
+  }
+  a.lut_mid = mid;
+  a.lut_lo = lo;
+  a.lut_hi = hi;
+}
+
+int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
+                 const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {
+  const ParamSet& p = ek->p;
+  Engine& E = engine();
+  int rc = ensure_acc(ek, count);
+  if (rc) return rc;
+  BootArgs a{};
+  a.bsk = ek->bsk_dev;
+  a.ksk = ek->ksk_dev;
+  const int f = p.is_goldilocks ? 1 : 0;
+  a.tw_fwd = E.tables.tw[f][0][0];
+  a.tw_inv = E.tables.tw[f][0][1];
+  a.c0 = c0;
+  a.c1 = c1;
+  a.c2 = c2;
+  a.lut = lut_dev;
+  a.out = out;
+  a.acc = ek->acc_dev;
+  a.qf = (double)p.q;
+  a.q8 = p.q >> 3;
+  a.n = p.n;
+  a.br_len = p.br_len;
+  a.br_bits = p.br_bits;
+  a.ks_len = p.ks_len;
+  a.ks_bits = p.ks_bits;
+  a.count = (uint32_t)count;
+  a.gate = gate;
+  lut_params(p, gate, a);
+  HIP_TRY(hipEventRecord(E.ev0, s));
+  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, s));
+  HIP_TRY(hipEventRecord(E.ev1, s));
+  return HB_OK;
+}
+
+int record_ms(hipStream_t s) {
+  (void)s;
+  Engine& E = engine();
+  HIP_TRY(hipEventSynchronize(E.ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+  E.last_ms = ms;
+  return HB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hb_params(int set, hb_params_t* out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  ParamSet p;
+  if (!param_set(set, &p)) return fail(HB_ERR_ARG, "unknown parameter set %d", set);
+  memset(out, 0, sizeof *out);
+  out->set = p.set;
+  out->is_goldilocks = p.is_goldilocks;
+  out->n = p.n;
+  out->N = p.N;
+  out->t = p.t;
+  out->br_bits = p.br_bits;
+  out->br_len = p.br_len;
+  out->ks_bits = p.ks_bits;
+  out->ks_len = p.ks_len;
+  out->q = p.q;
+  out->lwe_sigma = p.lwe_sigma;
+  out->ring_sigma = p.ring_sigma;
+  out->ks_sigma = p.ks_sigma;
+  return HB_OK;
+}
+
+int hb_key_words(int set, uint64_t* bw, uint64_t* kw) {
+  ParamSet p;
+  if (!param_set(set, &p)) return fail(HB_ERR_ARG, "unknown parameter set %d", set);
+  if (bw) *bw = bsk_words(p);
+  if (kw) *kw = ksk_words(p);
+  return HB_OK;
+}
+
+int hb_sk_create(int set, uint64_t seed, hb_sk_t** out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  auto* h = new (std::nothrow) hb_sk;
+  if (!h) return fail(HB_ERR_NOMEM, "allocation failed");
+  if (!secret_key_create(set, seed, &h->sk)) {
+    delete h;
+    return fail(HB_ERR_ARG, "unknown parameter set %d", set);
+  }
+  *out = h;
+  return HB_OK;
+}
+
+int hb_sk_destroy(hb_sk_t* sk) {
+  delete sk;
+  return HB_OK;
+}
+
+int hb_sk_export(const hb_sk_t* sk, uint64_t* lwe_sk, uint64_t* ring_sk) {
+  if (!sk) return fail(HB_ERR_ARG, "null sk");
+  if (lwe_sk) memcpy(lwe_sk, sk->sk.lwe_sk.data(), sk->sk.lwe_sk.size() * 8);
+  if (ring_sk) memcpy(ring_sk, sk->sk.ring_sk.data(), sk->sk.ring_sk.size() * 8);
+  return HB_OK;
+}
+
+int hb_encrypt(hb_sk_t* sk, uint32_t bit, uint64_t* out) {
+  if (!sk || !out) return fail(HB_ERR_ARG, "null argument");
+  if (bit >= sk->sk.p.t) return fail(HB_ERR_ARG, "message %u exceeds plaintext space", bit);
+  encrypt(sk->sk, bit, out);
+  return HB_OK;
+}
+
+int hb_decrypt(const hb_sk_t* sk, const uint64_t* c, uint32_t* bit, uint64_t* noise) {
+  if (!sk || !c || !bit) return fail(HB_ERR_ARG, "null argument");
+  decrypt(sk->sk, c, bit, noise);
+  return HB_OK;
+}
+
+int hb_ek_import(int set, const uint64_t* bsk, const uint64_t* ksk, int device, hb_ek_t** out) {
+  if (!bsk || !ksk || !out) return fail(HB_ERR_ARG, "null argument");
+  ParamSet p;
+  if (!param_set(set, &p)) return fail(HB_ERR_ARG, "unknown parameter set %d", set);
+  int rc = ensure_engine(device);
+  if (rc) return rc;
+  auto* ek = new (std::nothrow) hb_ek;
+  if (!ek) return fail(HB_ERR_NOMEM, "allocation failed");
+  ek->p = p;
+  ek->bsk_host.assign(bsk, bsk + bsk_words(p));
+  ek->ksk_host.assign(ksk, ksk + ksk_words(p));
+  rc = upload_keys(ek);
+  if (rc) {
+    hb_ek_destroy(ek);
+    return rc;
+  }
+  *out = ek;
+  return HB_OK;
+}
+
+int hb_ek_create(const hb_sk_t* sk, int device, hb_ek_t** out) {
+  if (!sk || !out) return fail(HB_ERR_ARG, "null argument");
+  int rc = ensure_engine(device);
+  if (rc) return rc;
+  auto* ek = new (std::nothrow) hb_ek;
+  if (!ek) return fail(HB_ERR_NOMEM, "allocation failed");
+  ek->p = sk->sk.p;
+  ek->bsk_host.resize(bsk_words(ek->p));
+  ek->ksk_host.resize(ksk_words(ek->p));
+  generate_bsk(sk->sk, ek->bsk_host.data());
+  generate_ksk(sk->sk, ek->ksk_host.data());
+  rc = upload_keys(ek);
+  if (rc) {
+    hb_ek_destroy(ek);
+    return rc;
+  }
+  *out = ek;
+  return HB_OK;
+}
+
+int hb_ek_export(const hb_ek_t* ek, uint64_t* bsk, uint64_t* ksk) {
+  if (!ek) return fail(HB_ERR_ARG, "null ek");
+  if (bsk) memcpy(bsk, ek->bsk_host.data(), ek->bsk_host.size() * 8);
+  if (ksk) memcpy(ksk, ek->ksk_host.data(), ek->ksk_host.size() * 8);
+  return HB_OK;
+}
+
+int hb_keygen_host(const hb_sk_t* sk, uint64_t* bsk, uint64_t* ksk) {
+  if (!sk) return fail(HB_ERR_ARG, "null sk");
+  if (bsk) generate_bsk(sk->sk, bsk);
+  if (ksk) generate_ksk(sk->sk, ksk);
+  return HB_OK;
+}
+
+int hb_ek_destroy(hb_ek_t* ek) {
+  if (!ek) return HB_OK;
+  if (ek->bsk_dev) (void)hipFree(ek->bsk_dev);
+  if (ek->ksk_dev) (void)hipFree(ek->ksk_dev);
+  if (ek->lut_dev) (void)hipFree(ek->lut_dev);
+  if (ek->io_dev) (void)hipFree(ek->io_dev);
+  if (ek->acc_dev) (void)hipFree(ek->acc_dev);
+  delete ek;
+  return HB_OK;
+}
+
+int hb_ek_params(const hb_ek_t* ek, hb_params_t* out) {
+  if (!ek) return fail(HB_ERR_ARG, "null ek");
+  return hb_params(ek->p.set, out);
+}
+
+int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
+                      const uint64_t* c2, uint64_t* outs, void* stream) {
+  if (!ek || !c0 || !outs) return fail(HB_ERR_ARG, "null argument");
+  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (!c1) return fail(HB_ERR_ARG, "gate needs two inputs");
+  if (gate == GATE_MAJORITY && !c2) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (count == 0) return HB_OK;
+  if (count > 0x7FFFFFFF) return fail(HB_ERR_ARG, "count too large");
+  hipStream_t s = stream ? (hipStream_t)stream : engine().stream;
+  std::lock_guard<std::mutex> g(ek->mu);
+  return launch_gates(ek, gate, count, c0, c1, c2, nullptr, outs, s);
+}
+
+int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,
+                  const uint64_t* c2s, uint64_t* outs) {
+  if (!ek || !c0s || !c1s || !outs) return fail(HB_ERR_ARG, "null argument");
+  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (gate == GATE_MAJORITY && !c2s) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (count == 0) return HB_OK;
+  Engine& E = engine();
+  std::lock_guard<std::mutex> g(ek->mu);
+  int rc = ensure_io(ek, count);
+  if (rc) return rc;
+  const size_t n1 = ek->p.n + 1, bytes = count * n1 * 8;
+  uint64_t* d0 = ek->io_dev;
+  uint64_t* d1 = d0 + count * n1;
+  uint64_t* d2 = d1 + count * n1;
+  uint64_t* dout = d2 + count * n1;
+  HIP_TRY(hipMemcpyAsync(d0, c0s, bytes, hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipMemcpyAsync(d1, c1s, bytes, hipMemcpyHostToDevice, E.stream));
+  if (c2s) HIP_TRY(hipMemcpyAsync(d2, c2s, bytes, hipMemcpyHostToDevice, E.stream));
+  rc = launch_gates(ek, gate, count, d0, d1, c2s ? d2 : nullptr, nullptr, dout, E.stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return record_ms(E.stream);
+}
+
+int hb_nand(hb_ek_t* ek, const uint64_t* c0, const uint64_t* c1, uint64_t* out) {
+  return hb_gate_batch(ek, GATE_NAND, 1, c0, c1, nullptr, out);
+}
+
+int hb_gate(hb_ek_t* ek, int gate, const uint64_t* c0, const uint64_t* c1, const uint64_t* c2,
+            uint64_t* out) {
+  return hb_gate_batch(ek, gate, 1, c0, c1, c2, out);
+}
+
+int hb_bootstrap(hb_ek_t* ek, const uint64_t* c, const uint64_t* lut, uint64_t* out) {
+  if (!ek || !c || !lut || !out) return fail(HB_ERR_ARG, "null argument");
+  Engine& E = engine();
+  std::lock_guard<std::mutex> g(ek->mu);
+  int rc = ensure_io(ek, 1);
+  if (rc) return rc;
+  const size_t n1 = ek->p.n + 1;
+  for (uint32_t k = 0; k < ek->p.N; k++)
+    if (lut[k] >= ek->p.q) return fail(HB_ERR_ARG, "LUT word %u not canonical", k);
+  uint64_t* d0 = ek->io_dev;
+  uint64_t* dout = d0 + 3 * n1;
+  HIP_TRY(hipMemcpyAsync(d0, c, n1 * 8, hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipMemcpyAsync(ek->lut_dev, lut, ek->p.N * 8, hipMemcpyHostToDevice, E.stream));
+  rc = launch_gates(ek, GATE_BOOTSTRAP, 1, d0, nullptr, nullptr, (const uint64_t*)ek->lut_dev,
+                    dout, E.stream);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, dout, n1 * 8, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return record_ms(E.stream);
+}
+
+// Evaluator::not (tfhe/src/bfhe/evaluate.rs:98-104): -c, then b += Q/4.
+int hb_not(const hb_ek_t* ek, const uint64_t* c, uint64_t* out) {
+  if (!ek || !c || !out) return fail(HB_ERR_ARG, "null argument");
+  const ParamSet& p = ek->p;
+  HostNtt ops;
+  ops.p = p;
+  for (uint32_t k = 0; k <= p.n; k++) out[k] = c[k] == 0 ? 0 : p.q - c[k];
+  out[p.n] = ops.add(out[p.n], p.q >> 2);
+  return HB_OK;
+}
+
+int hb_ntt_dev(int field, int inverse, uint32_t log_n, uint64_t* data, size_t count, void* stream) {
+  if (!data || (field != 0 && field != 1) || (log_n != 10 && log_n != 11))
+    return fail(HB_ERR_ARG, "bad ntt arguments");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  Engine& E = engine();
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  HIP_TRY(hipEventRecord(E.ev0, s));
+  HIP_TRY(launch_ntt(field, inverse, log_n, data, count, E.tables.tw[field][log_n - 10][inverse ? 1 : 0],
+                     E.tables.ninv[field][log_n - 10], s));
+  HIP_TRY(hipEventRecord(E.ev1, s));
+  return HB_OK;
+}
+
+int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t len,
+                            const uint64_t* rgsw_dev, uint64_t* rlwe_dev, size_t count,
+                            void* stream) {
+  if (!rgsw_dev || !rlwe_dev || (field != 0 && field != 1) || log_n != 10 || bits == 0 ||
+      bits > 16 || len == 0 || (uint64_t)bits * len < (field ? 64u : 31u))
+    return fail(HB_ERR_ARG, "bad external product arguments");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  Engine& E = engine();
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  // The kernel reads the RGSW in device word form with 1/N folded in: convert here
+  // into a cached buffer (one RGSW).
+  ParamSet p;
+  param_set(field ? SET_GOLDILOCKS : SET_BABYBEAR, &p);
+  const size_t words = (size_t)2 * len * 2 * 1024;
+  std::vector<uint64_t> host(words);
+  HIP_TRY(hipMemcpy(host.data(), rgsw_dev, words * 8, hipMemcpyDeviceToHost));
+  HostNtt ntt;
+  ntt.init(p, 10);
+  static void* conv = nullptr;
+  static size_t conv_bytes = 0;
+  const size_t wb = field ? 8 : 4;
+  if (conv_bytes < words * wb) {
+    if (conv) (void)hipFree(conv);
+    HIP_TRY(hipMalloc(&conv, words * wb));
+    conv_bytes = words * wb;
+  }
+  if (field) {
+    for (size_t k = 0; k < words; k++) host[k] = to_device_word(p, host[k], ntt.n_inv);
+    HIP_TRY(hipMemcpy(conv, host.data(), words * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<uint32_t> w(words);
+    for (size_t k = 0; k < words; k++) w[k] = (uint32_t)to_device_word(p, host[k], ntt.n_inv);
+    HIP_TRY(hipMemcpy(conv, w.data(), words * 4, hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipEventRecord(E.ev0, s));
+  HIP_TRY(launch_extprod(field, rlwe_dev, count, conv, len, bits, E.tables.tw[field][0][0],
+                         E.tables.tw[field][0][1], s));
+  HIP_TRY(hipEventRecord(E.ev1, s));
+  return HB_OK;
+}
+
+int hb_dev_alloc(size_t bytes, void** out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  HIP_TRY(hipMalloc(out, bytes));
+  return HB_OK;
+}
+int hb_dev_free(void* p) {
+  if (p) HIP_TRY(hipFree(p));
+  return HB_OK;
+}
+int hb_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return HB_OK;
+}
+int hb_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return HB_OK;
+}
+int hb_sync(void) {
+  Engine& E = engine();
+  if (E.stream) HIP_TRY(hipStreamSynchronize(E.stream));
+  HIP_TRY(hipDeviceSynchronize());
+  return HB_OK;
+}
+int hb_device_count(int* out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *out = n;
+  return HB_OK;
+}
+int hb_last_kernel_ms(double* out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  Engine& E = engine();
+  if (!E.ev1) return fail(HB_ERR_STATE, "no kernel launched yet");
+  HIP_TRY(hipEventSynchronize(E.ev1));
+  float ms = 0;
+  HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+  E.last_ms = ms;
+  *out = ms;
+  return HB_OK;
+}
+const char* hb_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,431 @@
+// HIP kernels for gfx950: fused gate bootstrapping (blind rotation + key switching +
+// sample extraction, one wavefront per gate) and the NTT / external-product
+// microbench kernels.
+//
+// Device layouts (all in HBM, written once by key upload):
+//   BSK  [i < n][c: c_neg_s_m, c_m][j < br_len][p: a, b][N slots]  — NTT domain,
+//        every word pre-multiplied by 1/N (so the inverse transform skips its
+//        normalisation) and, for BabyBear, by R = 2^32 (Montgomery constant form).
+//   KSK  [j < ks_len][p][N slots], same scaling.
+//   Slot order is the forward-NTT output order (bit-reversed evaluation order).
+//   Per-gate scratch: the accumulator RLWE, 2N words, raw Goldilocks words
+//   exactly as the reference's add_reduce produces them.
+#include "hb_kernels.hpp"
+#include "hb_ntt.hpp"
+
+namespace hb {
+
+template <class F>
+struct DevConst;
+template <>
+struct DevConst<Gl> {
+  using W = u64;
+};
+template <>
+struct DevConst<Bb> {
+  using W = u32;
+};
+
+// ----------------------------------------------------------------------------
+// Gate input combination (tfhe/src/bfhe/evaluate.rs:113-231) on canonical words,
+// returning the canonical value read by lwe_modulus_switch.
+// ----------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ typename DF<F>::W combine(int gate, typename DF<F>::W a,
+                                                     typename DF<F>::W b, typename DF<F>::W c) {
+  using W = typename DF<F>::W;
+  W v;
+  switch (gate) {
+    case GATE_XOR:
+    case GATE_XNOR: {
+      W d = F::sub(a, b);  // sub_component_wise_ref then double_inplace
+      v = F::add(d, d);
+      break;
+    }
+    case GATE_MAJORITY:
+      v = F::add(F::add(a, b), c);
+      break;
+    case GATE_BOOTSTRAP:
+      v = a;
+      break;
+    default:  // nand, and, or, nor: add_component_wise_ref (lattice/src/lwe.rs:72-81)
+      v = F::add(a, b);
+  }
+  return F::canon(v);
+}
+
+// lwe_modulus_switch (fhe_core/src/modulus_switch.rs:12-38): IEEE f64
+// round(v * 2N / Q), half away from zero, 2N -> 0.
+__device__ __forceinline__ u32 mod_switch(u64 v, double qf, u32 two_n) {
+  double r = ::round((double)v * (double)two_n / qf);
+  u32 t = (u32)(u64)r;
+  return t >= two_n ? t - two_n : t;
+}
+
+// LUT entry k (tfhe/src/bfhe/evaluate.rs:267-465, lut_step = 1), or the caller's LUT.
+template <class F>
+__device__ __forceinline__ typename DF<F>::W lut_at(const BootArgs& a, int k) {
+  using W = typename DF<F>::W;
+  if (a.lut) return (W)a.lut[k];
+  return k < (int)a.lut_mid ? (W)a.lut_lo : (W)a.lut_hi;
+}
+
+// Load 4 consecutive words (32 B Goldilocks / 16 B BabyBear) as one vector.
+__device__ __forceinline__ void load4(const u64* __restrict__ p, u64 (&v)[4]) {
+  const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+  ulonglong2 x = q[0], y = q[1];
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = y.x;
+  v[3] = y.y;
+}
+__device__ __forceinline__ void load4(const u32* __restrict__ p, u32 (&v)[4]) {
+  uint4 x = *reinterpret_cast<const uint4*>(p);
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = x.z;
+  v[3] = x.w;
+}
+
+// acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each.
+// ntt_add_mul_assign_fast (algebra/src/polynomial/univariate/ntt_polynomial.rs:620-629).
+template <class F, int E>
+__device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
+                                        const typename DF<F>::W (&x)[E],
+                                        const typename DF<F>::W* __restrict__ ka,
+                                        const typename DF<F>::W* __restrict__ kb, int lane) {
+  using W = typename DF<F>::W;
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int slot = 4 * (lane + 64 * q);
+    W va[4], vb[4];
+    load4(ka + slot, va);
+    load4(kb + slot, vb);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      pa[4 * q + r] = DF<F>::add(pa[4 * q + r], DF<F>::mul(x[4 * q + r], va[r]));
+      pb[4 * q + r] = DF<F>::add(pb[4 * q + r], DF<F>::mul(x[4 * q + r], vb[r]));
+    }
+  }
+}
+
+// Σ over one gadget of digit NTTs: for j < len, x = NTT(digit_j(src)), acc += x ⊙ key_j.
+// Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
+// values, digit_j = (v >> j*bits) & mask.
+template <class F, int E>
+__device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
+                                           const typename DF<F>::W (&src)[E], u32 len, u32 bits,
+                                           const typename DF<F>::W* __restrict__ key,
+                                           typename DF<F>::W* lds,
+                                           const typename DF<F>::W* __restrict__ tw, int lane) {
+  using W = typename DF<F>::W;
+  constexpr int N = 64 * E;
+  const W mask = (W)((1u << bits) - 1u);
+  for (u32 j = 0; j < len; j++) {
+    W x[E];
+    const u32 sh = j * bits;
+#pragma unroll
+    for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
+    ntt_fwd_wave<F, E>(x, lds, tw, lane);
+    const W* ka = key + (size_t)(2 * j) * N;
+    mac_row<F, E>(pa, pb, x, ka, ka + N, lane);
+  }
+}
+
+// Wave-wide sum mod P of one value per lane.
+template <class F>
+__device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    typename DF<F>::W o = __shfl_xor(v, off, 64);
+    v = DF<F>::add(v, o);
+  }
+  return v;
+}
+
+// ----------------------------------------------------------------------------
+// Fused bootstrap: one 64-thread workgroup (one wavefront) per gate.
+// EvaluationKey::bootstrap (tfhe/src/bfhe/evaluate.rs:42-60):
+//   lwe_modulus_switch → BinaryBlindRotationKey::blind_rotate (binary.rs:28-104)
+//   → acc.b[0] += Q/8 → KeySwitchingRLWEKey::key_switch_for_rlwe
+//   (fhe_core/src/key_switch/rlwe.rs:115-137) → extract_partial_lwe_locally.
+// ----------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
+                                               unsigned short* ms) {
+  using W = typename DF<F>::W;
+  constexpr int E = 16;
+  constexpr int N = 64 * E;
+  const int lane = threadIdx.x;
+  const u32 n = A.n, n1 = A.n + 1;
+  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
+  const W* __restrict__ itw = static_cast<const W*>(A.tw_inv);
+  const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
+  const W* __restrict__ ksk = static_cast<const W*>(A.ksk);
+  W* accA = static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  W* accB = accA + N;
+
+  // 1. gate combination + modulus switch of all n+1 coefficients.
+  const u64* c0 = A.c0 + (size_t)g * n1;
+  const u64* c1 = A.c1 ? A.c1 + (size_t)g * n1 : nullptr;
+  const u64* c2 = A.c2 ? A.c2 + (size_t)g * n1 : nullptr;
+  for (u32 k = lane; k < n1; k += 64) {
+    W a = (W)c0[k];
+    W b = c1 ? (W)c1[k] : (W)0;
+    W c = c2 ? (W)c2[k] : (W)0;
+    W v = combine<F>(A.gate, a, b, c);
+    ms[k] = (unsigned short)mod_switch((u64)v, A.qf, 2 * N);
+  }
+  wave_sync();
+
+  // 2. ACC = (0, LUT * X^-b)  (binary.rs:40-64)
+  const u32 bms = ms[n];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int k = lane + 64 * e;
+    W val;
+    if (bms == 0) {
+      val = lut_at<F>(A, k);
+    } else {
+      const int nb = 2 * N - (int)bms;
+      bool neg;
+      int src;
+      if (nb <= N) {
+        neg = k < nb;
+        src = neg ? k - nb + N : k - nb;
+      } else {
+        const int r = nb - N;
+        neg = k >= r;
+        src = neg ? k - r : k - r + N;
+      }
+      val = lut_at<F>(A, src);
+      if (neg) val = F::neg_raw(val);
+    }
+    accA[k] = 0;
+    accB[k] = val;
+  }
+  __syncthreads();
+
+  // 3. blind rotation: for a_i != 0, ACC += ExtProd((X^a_i - 1) ACC, RGSW(s_i)).
+  const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
+  for (u32 i = 0; i < n; i++) {
+    const int r = ms[i];
+    if (r == 0) continue;  // binary.rs:70
+    const W* key = bsk + (size_t)i * rgsw_words;
+    W pa[E], pb[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+#pragma unroll 1
+    for (int comp = 0; comp < 2; comp++) {
+      const W* acc = comp ? accB : accA;
+      // RLWE::mul_monic_monomial_sub_one_inplace (lattice/src/rlwe.rs:437-487):
+      // dst[k] = ±acc[k - r mod N] - acc[k], the ± being the raw-word negation.
+      W dst[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int k = lane + 64 * e;
+        bool neg;
+        int src;
+        if (r <= N) {
+          neg = k < r;
+          src = neg ? k - r + N : k - r;
+        } else {
+          const int rr = r - N;
+          neg = k >= rr;
+          src = neg ? k - rr : k - rr + N;
+        }
+        W v = acc[src];
+        if (neg) v = F::neg_raw(v);
+        dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
+      }
+      // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
+      gadget_mac<F, E>(pa, pb, dst, A.br_len, A.br_bits,
+                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, lane);
+    }
+    // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
+    // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
+    ntt_inv_wave<F, E>(pa, lds, itw, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = lane + 64 * e;
+      accA[k] = F::add(accA[k], pa[e]);
+    }
+    ntt_inv_wave<F, E>(pb, lds, itw, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = lane + 64 * e;
+      accB[k] = F::add(accB[k], pb[e]);
+    }
+    __syncthreads();
+  }
+
+  // 4. acc.b[0] += Q/8 (evaluate.rs:52) and key switching.
+  const W b0 = F::add(accB[0], (W)A.q8);
+  W src[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int k = lane + 64 * e;
+    src[e] = F::canon(F::neg_raw(accA[k]));  // polynomial.neg_assign() (rlwe.rs:1159)
+  }
+  W pa[E], pb[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+  gadget_mac<F, E>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, lane);
+  // b' = coefficient 0 of INTT(init.b + Σ) = b0 + Σ_slots Σ (1/N folded into the KSK).
+  W s = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) s = DF<F>::add(s, pb[e]);
+  s = wave_sum<F>(s);
+  const W bout = F::canon(DF<F>::add(F::canon(b0), s));
+  ntt_inv_wave<F, E>(pa, lds, itw, lane);
+  // extract_partial_lwe_locally (rlwe.rs:393-402): a' = [A0, -A_{N-1}, ..., -A_1][..n].
+  u64* out = A.out + (size_t)g * n1;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int p = lane + 64 * e;
+    if (p == 0) {
+      out[0] = (u64)pa[e];
+    } else {
+      const int k = N - p;
+      if (k < (int)n) out[k] = (u64)F::canon(F::neg_raw(pa[e]));
+    }
+  }
+  if (lane == 0) out[n] = (u64)bout;
+}
+
+template <class F>
+__global__ __launch_bounds__(64) void bootstrap_kernel(BootArgs A) {
+  using W = typename DF<F>::W;
+  __shared__ W lds[1024 + 64];
+  __shared__ unsigned short ms[1024];
+  bootstrap_gate<F>(A, blockIdx.x, lds, ms);
+}
+
+// ----------------------------------------------------------------------------
+// Microbench: negacyclic NTT over `count` polynomials, one wave per polynomial,
+// 4 waves per workgroup. Forward: natural order in, slot order out. Inverse: slot
+// order in, natural order out, times 1/N. Canonical u64 words in and out.
+// ----------------------------------------------------------------------------
+template <class F, int E>
+__global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t count, int inverse,
+                                                  const void* tw_, typename DF<F>::W ninv) {
+  using W = typename DF<F>::W;
+  constexpr int N = 64 * E;
+  __shared__ W lds_all[4][N + N / 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t poly = (size_t)blockIdx.x * 4 + wave;
+  if (poly >= count) return;
+  const W* tw = static_cast<const W*>(tw_);
+  W* lds = lds_all[wave];
+  u64* d = data + poly * N;
+  W x[E];
+  if (!inverse) {
+#pragma unroll
+    for (int e = 0; e < E; e++) x[e] = (W)d[lane + 64 * e];
+    ntt_fwd_wave<F, E>(x, lds, tw, lane);
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) {
+      const int p = 4 * (lane + 64 * q);
+      ulonglong2* o = reinterpret_cast<ulonglong2*>(d + p);
+      o[0] = make_ulonglong2((u64)DF<F>::canon(x[4 * q]), (u64)DF<F>::canon(x[4 * q + 1]));
+      o[1] = make_ulonglong2((u64)DF<F>::canon(x[4 * q + 2]), (u64)DF<F>::canon(x[4 * q + 3]));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) {
+      const int p = 4 * (lane + 64 * q);
+      const ulonglong2* s = reinterpret_cast<const ulonglong2*>(d + p);
+      ulonglong2 a = s[0], b = s[1];
+      x[4 * q] = (W)a.x;
+      x[4 * q + 1] = (W)a.y;
+      x[4 * q + 2] = (W)b.x;
+      x[4 * q + 3] = (W)b.y;
+    }
+    ntt_inv_wave<F, E>(x, lds, tw, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) d[lane + 64 * e] = (u64)DF<F>::canon(DF<F>::mul(x[e], ninv));
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Microbench: external product of `count` RLWEs (a then b, 2N canonical words each)
+// with one RGSW in the device layout. One wave per RLWE, 4 waves per workgroup.
+// ----------------------------------------------------------------------------
+template <class F>
+__global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, size_t count,
+                                                      const void* rgsw_, u32 len, u32 bits,
+                                                      const void* tw_, const void* itw_) {
+  using W = typename DF<F>::W;
+  constexpr int E = 16, N = 1024;
+  __shared__ W lds_all[4][N + N / 16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t idx = (size_t)blockIdx.x * 4 + wave;
+  if (idx >= count) return;
+  const W* rgsw = static_cast<const W*>(rgsw_);
+  const W* tw = static_cast<const W*>(tw_);
+  const W* itw = static_cast<const W*>(itw_);
+  W* lds = lds_all[wave];
+  u64* d = rlwe + idx * 2 * N;
+  W pa[E], pb[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+#pragma unroll 1
+  for (int comp = 0; comp < 2; comp++) {
+    W src[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) src[e] = (W)d[comp * N + lane + 64 * e];
+    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, tw, lane);
+  }
+  ntt_inv_wave<F, E>(pa, lds, itw, lane);
+  ntt_inv_wave<F, E>(pb, lds, itw, lane);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    d[lane + 64 * e] = (u64)pa[e];
+    d[N + lane + 64 * e] = (u64)pb[e];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Launchers
+// ----------------------------------------------------------------------------
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s) {
+  dim3 grid(a.count), block(64);
+  if (is_goldilocks)
+    hipLaunchKernelGGL(bootstrap_kernel<Gl>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(bootstrap_kernel<Bb>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_ntt(int is_goldilocks, int inverse, u32 log_n, u64* data, size_t count,
+                      const void* tw, u64 ninv, hipStream_t s) {
+  dim3 grid((unsigned)((count + 3) / 4)), block(256);
+  if (is_goldilocks) {
+    if (log_n == 10)
+      hipLaunchKernelGGL((ntt_kernel<Gl, 16>), grid, block, 0, s, data, count, inverse, tw, (u64)ninv);
+    else if (log_n == 11)
+      hipLaunchKernelGGL((ntt_kernel<Gl, 32>), grid, block, 0, s, data, count, inverse, tw, (u64)ninv);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    if (log_n == 10)
+      hipLaunchKernelGGL((ntt_kernel<Bb, 16>), grid, block, 0, s, data, count, inverse, tw, (u32)ninv);
+    else if (log_n == 11)
+      hipLaunchKernelGGL((ntt_kernel<Bb, 32>), grid, block, 0, s, data, count, inverse, tw, (u32)ninv);
+    else
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_extprod(int is_goldilocks, u64* rlwe, size_t count, const void* rgsw, u32 len,
+                          u32 bits, const void* tw, const void* itw, hipStream_t s) {
+  dim3 grid((unsigned)((count + 3) / 4)), block(256);
+  if (is_goldilocks)
+    hipLaunchKernelGGL(extprod_kernel<Gl>, grid, block, 0, s, rlwe, count, rgsw, len, bits, tw, itw);
+  else
+    hipLaunchKernelGGL(extprod_kernel<Bb>, grid, block, 0, s, rlwe, count, rgsw, len, bits, tw, itw);
+  return hipGetLastError();
+}
+
+}  // namespace hb

@@ -1,0 +1,48 @@
+// Kernel-side argument block and launchers (hb_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace hb {
+
+enum GateKind : int {
+  GATE_NAND = 0,
+  GATE_AND = 1,
+  GATE_OR = 2,
+  GATE_NOR = 3,
+  GATE_XOR = 4,
+  GATE_XNOR = 5,
+  GATE_MAJORITY = 6,
+  GATE_BOOTSTRAP = 7  // c0 is the ciphertext, `lut` the test polynomial
+};
+
+struct BootArgs {
+  const void* bsk;     // device layout, see hb_kernels.hip
+  const void* ksk;
+  const void* tw_fwd;  // psi^bitrev(k), N words
+  const void* tw_inv;  // psi^-bitrev(k), N words
+  const uint64_t* c0;  // count*(n+1) canonical words
+  const uint64_t* c1;  // may be null for GATE_BOOTSTRAP
+  const uint64_t* c2;  // majority only
+  const uint64_t* lut; // null: built from (lut_mid, lut_lo, lut_hi)
+  uint64_t* out;       // count*(n+1) canonical words
+  void* acc;           // count * 2N words of scratch
+  double qf;           // (double)Q, as the reference's `as f64`
+  uint64_t q8;         // Q >> 3 in the device word form
+  uint64_t lut_lo, lut_hi;
+  uint32_t lut_mid;
+  uint32_t n, br_len, br_bits, ks_len, ks_bits;
+  uint32_t count;
+  int gate;
+};
+
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s);
+hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
+                      const void* tw, uint64_t ninv, hipStream_t s);
+hipError_t launch_extprod(int is_goldilocks, uint64_t* rlwe, size_t count, const void* rgsw,
+                          uint32_t len, uint32_t bits, const void* tw, const void* itw,
+                          hipStream_t s);
+
+}  // namespace hb

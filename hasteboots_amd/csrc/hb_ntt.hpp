@@ -1,0 +1,232 @@
+// Wave-level negacyclic NTT for gfx950 (wave64), shared by the bootstrap and
+// microbench kernels.
+//
+// One wavefront owns one polynomial of N = 64*E words: lane l holds E words in
+// registers. The transform is the Cooley-Tukey / Gentleman-Sande pair whose
+// forward output slot i is X(psi^(2*bitrev(i)+1)) (bit-reversed evaluation order),
+// split into three register passes joined by two exchanges through a padded LDS
+// buffer of N + N/16 words:
+//   pass 1: log2(E) stages; lane l holds positions l + 64e (twiddles are uniform
+//           across the wave, so they come from the scalar cache);
+//   pass 2: 4 stages inside each 64-word block; lane holds positions
+//           64b + j0 + 4k of block b = u>>2, j0 = u&3 for u = l + 64v;
+//   pass 3: 2 stages inside each 4-word block; lane holds positions 4(l+64q)+r,
+//           i.e. 4 consecutive words, so global loads/stores of the NTT-domain
+//           layout are 32-byte (Goldilocks) or 16-byte (BabyBear) vectors per lane.
+// The inverse runs the passes in reverse and takes/gives the same layouts, so the
+// NTT-domain data never needs a permutation between kernels and keys.
+//
+// The reference delegates these transforms to concrete-ntt
+// (algebra/src/transformation/concrete.rs:51-61, 138-148).
+#pragma once
+#include "hb_field.hpp"
+
+namespace hb {
+
+// Intra-wave LDS ordering: all lanes of one wavefront execute LDS operations in
+// order; the fences stop the compiler from moving accesses across the exchange.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lds_pad(int p) { return p + ((p >> 6) << 2); }
+
+template <class F>
+struct DF;
+
+template <>
+struct DF<Gl> {
+  using W = u64;
+  __device__ __forceinline__ static W add(W a, W b) { return Gl::add(a, b); }
+  __device__ __forceinline__ static W sub(W a, W b) { return Gl::sub(a, b); }
+  __device__ __forceinline__ static W mul(W a, W c) { return Gl::mul(a, c); }
+  __device__ __forceinline__ static W canon(W a) { return Gl::canon(a); }
+};
+
+template <>
+struct DF<Bb> {
+  using W = u32;
+  __device__ __forceinline__ static W add(W a, W b) { return Bb::add(a, b); }
+  __device__ __forceinline__ static W sub(W a, W b) { return Bb::sub(a, b); }
+  // Constants (twiddles, keys) are stored times R = 2^32.
+  __device__ __forceinline__ static W mul(W a, W c) { return Bb::mont_mul(a, c); }
+  __device__ __forceinline__ static W canon(W a) { return a; }
+};
+
+template <class F>
+__device__ __forceinline__ void ct_bfly(typename DF<F>::W& x, typename DF<F>::W& y,
+                                        typename DF<F>::W w) {
+  auto t = DF<F>::mul(y, w);
+  y = DF<F>::sub(x, t);
+  x = DF<F>::add(x, t);
+}
+
+template <class F>
+__device__ __forceinline__ void gs_bfly(typename DF<F>::W& x, typename DF<F>::W& y,
+                                        typename DF<F>::W w) {
+  auto s = DF<F>::add(x, y);
+  y = DF<F>::mul(DF<F>::sub(x, y), w);
+  x = s;
+}
+
+template <int E>
+struct Log2 {
+  static constexpr int value = E == 16 ? 4 : E == 32 ? 5 : E == 8 ? 3 : -1;
+};
+
+// Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
+// tw = psi_rev (psi^bitrev(k), constants in the field's constant form).
+template <class F, int E>
+__device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
+                                             const typename DF<F>::W* __restrict__ tw, int lane) {
+  using W = typename DF<F>::W;
+  constexpr int LOGE = Log2<E>::value;
+  static_assert(LOGE > 0, "E must be 8, 16 or 32");
+  // pass 1
+#pragma unroll
+  for (int s = 0; s < LOGE; s++) {
+    const int m = 1 << s;
+    const int half = (E / 2) >> s;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (e & half) continue;
+      const W w = tw[m + e / (2 * half)];
+      ct_bfly<F>(x[e], x[e + half], w);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) lds[lds_pad(lane + 64 * e)] = x[e];
+  wave_sync();
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int u = lane + 64 * v;
+    const int base = 64 * (u >> 2) + (u & 3);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[lds_pad(base + 4 * k)];
+  }
+  wave_sync();
+  // pass 2
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int b = (lane + 64 * v) >> 2;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int m = E << s;
+      const int half = 8 >> s;
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        if (k & half) continue;
+        const W w = tw[m + (b << s) + k / (2 * half)];
+        ct_bfly<F>(x[16 * v + k], x[16 * v + k + half], w);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int u = lane + 64 * v;
+    const int base = 64 * (u >> 2) + (u & 3);
+#pragma unroll
+    for (int k = 0; k < 16; k++) lds[lds_pad(base + 4 * k)] = x[16 * v + k];
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int p = 4 * (lane + 64 * q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) x[4 * q + r] = lds[lds_pad(p) + r];
+  }
+  wave_sync();
+  // pass 3
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int c = lane + 64 * q;
+    const W w0 = tw[16 * E + c];
+    ct_bfly<F>(x[4 * q + 0], x[4 * q + 2], w0);
+    ct_bfly<F>(x[4 * q + 1], x[4 * q + 3], w0);
+    const W w1 = tw[32 * E + 2 * c];
+    const W w2 = tw[32 * E + 2 * c + 1];
+    ct_bfly<F>(x[4 * q + 0], x[4 * q + 1], w1);
+    ct_bfly<F>(x[4 * q + 2], x[4 * q + 3], w2);
+  }
+}
+
+// Inverse transform without the 1/N factor. In: slot layout of ntt_fwd_wave.
+// Out: x[e] = position lane + 64e, canonical. itw = psi^-bitrev(k).
+template <class F, int E>
+__device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
+                                             const typename DF<F>::W* __restrict__ itw, int lane) {
+  using W = typename DF<F>::W;
+  constexpr int LOGE = Log2<E>::value;
+  // pass 3
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int c = lane + 64 * q;
+    const W w1 = itw[32 * E + 2 * c];
+    const W w2 = itw[32 * E + 2 * c + 1];
+    gs_bfly<F>(x[4 * q + 0], x[4 * q + 1], w1);
+    gs_bfly<F>(x[4 * q + 2], x[4 * q + 3], w2);
+    const W w0 = itw[16 * E + c];
+    gs_bfly<F>(x[4 * q + 0], x[4 * q + 2], w0);
+    gs_bfly<F>(x[4 * q + 1], x[4 * q + 3], w0);
+  }
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int p = 4 * (lane + 64 * q);
+#pragma unroll
+    for (int r = 0; r < 4; r++) lds[lds_pad(p) + r] = x[4 * q + r];
+  }
+  wave_sync();
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int u = lane + 64 * v;
+    const int base = 64 * (u >> 2) + (u & 3);
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[lds_pad(base + 4 * k)];
+  }
+  wave_sync();
+  // pass 2
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int b = (lane + 64 * v) >> 2;
+#pragma unroll
+    for (int s = 3; s >= 0; s--) {
+      const int m = E << s;
+      const int half = 8 >> s;
+#pragma unroll
+      for (int k = 0; k < 16; k++) {
+        if (k & half) continue;
+        const W w = itw[m + (b << s) + k / (2 * half)];
+        gs_bfly<F>(x[16 * v + k], x[16 * v + k + half], w);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < E / 16; v++) {
+    const int u = lane + 64 * v;
+    const int base = 64 * (u >> 2) + (u & 3);
+#pragma unroll
+    for (int k = 0; k < 16; k++) lds[lds_pad(base + 4 * k)] = x[16 * v + k];
+  }
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < E; e++) x[e] = lds[lds_pad(lane + 64 * e)];
+  wave_sync();
+  // pass 1
+#pragma unroll
+  for (int s = LOGE - 1; s >= 0; s--) {
+    const int m = 1 << s;
+    const int half = (E / 2) >> s;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (e & half) continue;
+      const W w = itw[m + e / (2 * half)];
+      gs_bfly<F>(x[e], x[e + half], w);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < E; e++) x[e] = DF<F>::canon(x[e]);
+}
+
+}  // namespace hb

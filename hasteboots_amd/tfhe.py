@@ -1,0 +1,291 @@
+"""Host-side mirror of the reference `tfhe` crate's gate API over the C ABI.
+
+Reference interface (tfhe/src/lib.rs:1-9):
+  KeyGen::generate_secret_key(params) -> SecretKeyPack         tfhe/src/key_gen.rs:12
+  Encryptor::new(sk).encrypt(m) -> LWECiphertext               tfhe/src/encrypt.rs:14,20
+  Decryptor::new(sk).decrypt(c) / decrypt_with_noise(c)        tfhe/src/decrypt.rs:20,26
+  Evaluator::new(&sk) ; nand/and/or/nor/xor/xnor/majority/mux/not/bootstrap
+                                                               tfhe/src/bfhe/evaluate.rs:72-263
+Parameter constants: tfhe/src/bfhe/parameters.rs:45-99.
+
+Differences, by design: key generation takes a seed (the reference draws from OS
+entropy), and every Evaluator method has a batched form that runs many independent
+gates in one GPU launch. Errors raise HbError instead of panicking.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._lib import HbParams, check, lib, ptr
+
+SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA = 0, 1, 2
+GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6}
+
+
+@dataclass(frozen=True)
+class Parameters:
+    """Parameters<Q> (fhe_core/src/parameter.rs:71-77) for one of the named sets."""
+    set_id: int
+    n: int
+    N: int
+    t: int
+    q: int
+    br_bits: int
+    br_len: int
+    ks_bits: int
+    ks_len: int
+    is_goldilocks: bool
+
+    @staticmethod
+    def of(set_id: int) -> "Parameters":
+        p = HbParams()
+        check(lib().hb_params(set_id, C.byref(p)))
+        return Parameters(p.set, p.n, p.N, p.t, p.q, p.br_bits, p.br_len, p.ks_bits, p.ks_len,
+                          bool(p.is_goldilocks))
+
+    def lwe_dimension(self) -> int:
+        return self.n
+
+    def ring_dimension(self) -> int:
+        return self.N
+
+    def lwe_cipher_modulus_value(self) -> int:
+        return self.q
+
+
+def BABYBEAR_BINARY_128_BITS_PARAMETERS() -> Parameters:
+    return Parameters.of(SET_BABYBEAR)
+
+
+def GOLDILOCKS_BINARY_128_BITS_PARAMETERS() -> Parameters:
+    return Parameters.of(SET_GOLDILOCKS)
+
+
+def ZAMA_BINARY_128_BITS_PARAMETERS() -> Parameters:
+    return Parameters.of(SET_ZAMA)
+
+
+class LWECiphertext:
+    """LWE<Q> { a: Vec<Q>, b: Q } (lattice/src/lwe.rs:20-23), stored as n+1 canonical words."""
+
+    __slots__ = ("words",)
+
+    def __init__(self, words: np.ndarray):
+        self.words = np.ascontiguousarray(words, dtype=np.uint64)
+
+    @property
+    def a(self) -> np.ndarray:
+        return self.words[:-1]
+
+    @property
+    def b(self) -> int:
+        return int(self.words[-1])
+
+    def __eq__(self, other) -> bool:
+        return isinstance(other, LWECiphertext) and np.array_equal(self.words, other.words)
+
+
+class SecretKeyPack:
+    """SecretKeyPack<Q> (fhe_core/src/secret_key.rs:42-56). Not thread-safe (holds a PRG)."""
+
+    def __init__(self, params: Parameters, seed: int = 0):
+        self.params = params
+        self.seed = seed
+        h = C.c_void_p()
+        check(lib().hb_sk_create(params.set_id, C.c_uint64(seed), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().hb_sk_destroy(h)
+            self._h = None
+
+    def lwe_secret_key(self) -> np.ndarray:
+        out = np.zeros(self.params.n, dtype=np.uint64)
+        check(lib().hb_sk_export(self._h, ptr(out), None))
+        return out
+
+    def ring_secret_key(self) -> np.ndarray:
+        out = np.zeros(self.params.N, dtype=np.uint64)
+        check(lib().hb_sk_export(self._h, None, ptr(out)))
+        return out
+
+    def parameters(self) -> Parameters:
+        return self.params
+
+    def generate_keys_host(self) -> tuple[np.ndarray, np.ndarray]:
+        """BSK and KSK in the canonical layout, generated on the host (no GPU needed)."""
+        bw, kw = C.c_uint64(), C.c_uint64()
+        check(lib().hb_key_words(self.params.set_id, C.byref(bw), C.byref(kw)))
+        bsk = np.zeros(bw.value, dtype=np.uint64)
+        ksk = np.zeros(kw.value, dtype=np.uint64)
+        check(lib().hb_keygen_host(self._h, ptr(bsk), ptr(ksk)))
+        return bsk, ksk
+
+
+class KeyGen:
+    @staticmethod
+    def generate_secret_key(params: Parameters, seed: int = 0) -> SecretKeyPack:
+        return SecretKeyPack(params, seed)
+
+
+class Encryptor:
+    def __init__(self, sk: SecretKeyPack):
+        self.sk = sk
+
+    def encrypt(self, m: int) -> LWECiphertext:
+        out = np.zeros(self.sk.params.n + 1, dtype=np.uint64)
+        check(lib().hb_encrypt(self.sk._h, int(m), ptr(out)))
+        return LWECiphertext(out)
+
+    def encrypt_many(self, bits) -> np.ndarray:
+        bits = np.asarray(bits, dtype=np.uint32)
+        out = np.zeros((len(bits), self.sk.params.n + 1), dtype=np.uint64)
+        for i, m in enumerate(bits):
+            check(lib().hb_encrypt(self.sk._h, int(m), ptr(out[i])))
+        return out
+
+
+class Decryptor:
+    def __init__(self, sk: SecretKeyPack):
+        self.sk = sk
+
+    def decrypt(self, c: LWECiphertext) -> int:
+        return self.decrypt_with_noise(c)[0]
+
+    def decrypt_with_noise(self, c) -> tuple[int, int]:
+        words = c.words if isinstance(c, LWECiphertext) else np.ascontiguousarray(c, np.uint64)
+        bit = C.c_uint32()
+        noise = C.c_uint64()
+        check(lib().hb_decrypt(self.sk._h, ptr(words), C.byref(bit),
+                               C.cast(C.byref(noise), C.POINTER(C.c_uint64))))
+        return int(bit.value), int(noise.value)
+
+
+class Evaluator:
+    """Evaluator<Q> (tfhe/src/bfhe/evaluate.rs:63-263) backed by keys resident in HBM."""
+
+    def __init__(self, sk: SecretKeyPack | None = None, device: int = 0, *, keys=None,
+                 params: Parameters | None = None):
+        h = C.c_void_p()
+        if sk is not None:
+            self.params = sk.params
+            check(lib().hb_ek_create(sk._h, device, C.byref(h)))
+        else:
+            bsk, ksk = keys
+            self.params = params
+            bsk = np.ascontiguousarray(bsk, np.uint64)
+            ksk = np.ascontiguousarray(ksk, np.uint64)
+            check(lib().hb_ek_import(params.set_id, ptr(bsk), ptr(ksk), device, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().hb_ek_destroy(h)
+            self._h = None
+
+    def parameters(self) -> Parameters:
+        return self.params
+
+    def export_keys(self) -> tuple[np.ndarray, np.ndarray]:
+        bw, kw = C.c_uint64(), C.c_uint64()
+        check(lib().hb_key_words(self.params.set_id, C.byref(bw), C.byref(kw)))
+        bsk = np.zeros(bw.value, dtype=np.uint64)
+        ksk = np.zeros(kw.value, dtype=np.uint64)
+        check(lib().hb_ek_export(self._h, ptr(bsk), ptr(ksk)))
+        return bsk, ksk
+
+    def _gate(self, gate: str, c0, c1, c2=None) -> LWECiphertext:
+        out = np.zeros(self.params.n + 1, dtype=np.uint64)
+        check(lib().hb_gate(self._h, GATES[gate], ptr(c0.words), ptr(c1.words),
+                            ptr(c2.words) if c2 is not None else None, ptr(out)))
+        return LWECiphertext(out)
+
+    def nand(self, c0: LWECiphertext, c1: LWECiphertext) -> LWECiphertext:
+        out = np.zeros(self.params.n + 1, dtype=np.uint64)
+        check(lib().hb_nand(self._h, ptr(c0.words), ptr(c1.words), ptr(out)))
+        return LWECiphertext(out)
+
+    def and_(self, c0, c1):
+        return self._gate("and", c0, c1)
+
+    def or_(self, c0, c1):
+        return self._gate("or", c0, c1)
+
+    def nor(self, c0, c1):
+        return self._gate("nor", c0, c1)
+
+    def xor(self, c0, c1):
+        return self._gate("xor", c0, c1)
+
+    def xnor(self, c0, c1):
+        return self._gate("xnor", c0, c1)
+
+    def majority(self, c0, c1, c2):
+        return self._gate("majority", c0, c1, c2)
+
+    def not_(self, c: LWECiphertext) -> LWECiphertext:
+        out = np.zeros(self.params.n + 1, dtype=np.uint64)
+        check(lib().hb_not(self._h, ptr(c.words), ptr(out)))
+        return LWECiphertext(out)
+
+    def mux(self, c0, c1, c2):
+        """evaluate.rs:240-263: (c0 AND c1) + (NOT c0 AND c2), then OR-bootstrap."""
+        nc0 = self.not_(c0)
+        t = self.gate_batch("and", np.stack([c0.words, nc0.words]),
+                            np.stack([c1.words, c2.words]))
+        q = self.params.q
+        s = ((t[0].astype(object) + t[1].astype(object)) % q).astype(np.uint64)
+        lut = self.lut("or")
+        return self.bootstrap(LWECiphertext(s), lut)
+
+    def lut(self, kind: str) -> np.ndarray:
+        """Test polynomials of evaluate.rs:267-465 (lut_step = 1), canonical words."""
+        N, q = self.params.N, self.params.q
+        q8 = q >> 3
+        pos, neg = q8, q - q8
+        mid, lo, hi = {
+            "nand": ((N >> 1) + (N >> 2), pos, neg), "and": ((N >> 1) + (N >> 2), neg, pos),
+            "majority": ((N >> 1) + (N >> 2), neg, pos), "or": (N >> 2, neg, pos),
+            "nor": (N >> 2, pos, neg), "xor": (N >> 1, neg, pos), "xnor": (N >> 1, pos, neg),
+        }[kind]
+        v = np.full(N, hi, dtype=np.uint64)
+        v[:mid] = lo
+        return v
+
+    def bootstrap(self, c: LWECiphertext, lut: np.ndarray) -> LWECiphertext:
+        out = np.zeros(self.params.n + 1, dtype=np.uint64)
+        lut = np.ascontiguousarray(lut, np.uint64)
+        check(lib().hb_bootstrap(self._h, ptr(c.words), ptr(lut), ptr(out)))
+        return LWECiphertext(out)
+
+    def gate_batch(self, gate: str, c0s: np.ndarray, c1s: np.ndarray,
+                   c2s: np.ndarray | None = None) -> np.ndarray:
+        """`count` independent gates in one launch; arrays of shape (count, n+1)."""
+        c0s = np.ascontiguousarray(c0s, np.uint64)
+        c1s = np.ascontiguousarray(c1s, np.uint64)
+        c2s = None if c2s is None else np.ascontiguousarray(c2s, np.uint64)
+        out = np.zeros_like(c0s)
+        check(lib().hb_gate_batch(self._h, GATES[gate], c0s.shape[0], ptr(c0s), ptr(c1s),
+                                  ptr(c2s), ptr(out)))
+        return out
+
+    def nand_batch(self, c0s: np.ndarray, c1s: np.ndarray) -> np.ndarray:
+        return self.gate_batch("nand", c0s, c1s)
+
+
+def last_kernel_ms() -> float:
+    v = C.c_double()
+    check(lib().hb_last_kernel_ms(C.byref(v)))
+    return v.value
+
+
+def device_count() -> int:
+    n = C.c_int()
+    check(lib().hb_device_count(C.byref(n)))
+    return n.value

@@ -1,0 +1,160 @@
+/*
+ * hb.h — C ABI of the MI355X TFHE gate-bootstrapping engine (libhasteboots.so).
+ *
+ * This is the drop-in boundary for the reference `tfhe` crate's gate API
+ * (tfhe/src/lib.rs:1-9). Each entry point names the reference interface it
+ * replaces. Conventions:
+ *   - Every function returns an hb_status (0 = HB_OK); the reference panics or
+ *     returns Err where these return a negative code.
+ *   - Words are little-endian uint64_t canonical field values (in [0, Q)), for all
+ *     parameter sets. An LWE ciphertext is n+1 words: a[0..n) then b.
+ *   - Handles are opaque and library-owned. hb_ek_t is read-only after creation
+ *     and may be shared across threads; hb_sk_t carries an encryption PRG and is
+ *     not thread-safe (the reference keeps its PRG in a RefCell,
+ *     fhe_core/src/secret_key.rs:55).
+ *   - Entry points with a `_dev` suffix take device pointers on the engine's GPU
+ *     and an optional stream (NULL = the engine's own stream); all others take
+ *     host pointers and return after the result is on the host.
+ */
+#ifndef HB_H_
+#define HB_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HB_OK = 0,
+  HB_ERR_ARG = -1,       /* invalid argument (null pointer, bad set id, bad size) */
+  HB_ERR_DEVICE = -2,    /* no usable GPU, or a HIP runtime call failed */
+  HB_ERR_NOMEM = -3,     /* host or device allocation failed */
+  HB_ERR_CAPACITY = -4,  /* request exceeds a configured limit (serving admission) */
+  HB_ERR_STATE = -5      /* call not valid in the handle's current state */
+} hb_status;
+
+/* Parameter set identifiers: tfhe/src/bfhe/parameters.rs:45-99. */
+typedef enum {
+  HB_SET_BABYBEAR = 0,   /* BABYBEAR_BINARY_128_BITS_PARAMETERS */
+  HB_SET_GOLDILOCKS = 1, /* GOLDILOCKS_BINARY_128_BITS_PARAMETERS */
+  HB_SET_ZAMA = 2        /* ZAMA_BINARY_128_BITS_PARAMETERS (= vfhe ZAMA_GOLDILOCKS_PARAMETERS) */
+} hb_set_id;
+
+/* Gate kinds: tfhe::Evaluator::{nand, and, or, nor, xor, xnor, majority}
+ * (tfhe/src/bfhe/evaluate.rs:113-231). */
+typedef enum {
+  HB_GATE_NAND = 0,
+  HB_GATE_AND = 1,
+  HB_GATE_OR = 2,
+  HB_GATE_NOR = 3,
+  HB_GATE_XOR = 4,
+  HB_GATE_XNOR = 5,
+  HB_GATE_MAJORITY = 6
+} hb_gate_kind;
+
+typedef struct {
+  int32_t set;
+  int32_t is_goldilocks;
+  uint32_t n;       /* lwe_dimension */
+  uint32_t N;       /* ring_dimension */
+  uint32_t t;       /* lwe_plain_modulus */
+  uint32_t br_bits; /* blind_rotation_basis_bits */
+  uint32_t br_len;  /* decompose_len of the blind-rotation basis */
+  uint32_t ks_bits; /* key_switching_basis_bits */
+  uint32_t ks_len;  /* decompose_len of the key-switching basis */
+  uint32_t pad_;
+  uint64_t q;       /* ring_modulus = lwe cipher modulus */
+  double lwe_sigma, ring_sigma, ks_sigma;
+} hb_params_t;
+
+typedef struct hb_sk hb_sk_t;
+typedef struct hb_ek hb_ek_t;
+
+/* Parameters::new via the static constants (fhe_core/src/parameter.rs:111-172). */
+int hb_params(int set, hb_params_t* out);
+
+/* Sizes in words of the canonical key layouts used by hb_ek_export / hb_ek_import:
+ * BSK [i<n][c_neg_s_m, c_m][j<br_len][a, b][N], KSK [j<ks_len][a, b][N], both in the
+ * NTT domain (slot i = evaluation at psi^(2*bitrev(i)+1)). */
+int hb_key_words(int set, uint64_t* bsk_words, uint64_t* ksk_words);
+
+/* KeyGen::generate_secret_key (tfhe/src/key_gen.rs:12) → SecretKeyPack::new
+ * (fhe_core/src/secret_key.rs:67-89), deterministic from `seed` (the reference seeds
+ * from OS entropy). */
+int hb_sk_create(int set, uint64_t seed, hb_sk_t** out);
+int hb_sk_destroy(hb_sk_t* sk);
+/* lwe_secret_key (n words) and ring_secret_key (N words), canonical. */
+int hb_sk_export(const hb_sk_t* sk, uint64_t* lwe_sk, uint64_t* ring_sk);
+
+/* Encryptor::encrypt (tfhe/src/encrypt.rs:20) → SecretKeyPack::encrypt
+ * (fhe_core/src/secret_key.rs:129-150). Draws from the key's encryption stream. */
+int hb_encrypt(hb_sk_t* sk, uint32_t bit, uint64_t* out_lwe);
+/* Decryptor::decrypt / decrypt_with_noise (tfhe/src/decrypt.rs:20,26). `noise` may be NULL. */
+int hb_decrypt(const hb_sk_t* sk, const uint64_t* lwe, uint32_t* bit, uint64_t* noise);
+
+/* Evaluator::new (tfhe/src/bfhe/evaluate.rs:72-76) → EvaluationKey::new (:26-39):
+ * generates BSK + KSK on the host and makes them resident in HBM of `device`. */
+int hb_ek_create(const hb_sk_t* sk, int device, hb_ek_t** out);
+/* Same, from keys in the canonical layout (e.g. produced elsewhere). */
+int hb_ek_import(int set, const uint64_t* bsk, const uint64_t* ksk, int device, hb_ek_t** out);
+int hb_ek_export(const hb_ek_t* ek, uint64_t* bsk, uint64_t* ksk);
+/* Host-only key generation into caller buffers (canonical layout), no device needed. */
+int hb_keygen_host(const hb_sk_t* sk, uint64_t* bsk, uint64_t* ksk);
+int hb_ek_destroy(hb_ek_t* ek);
+int hb_ek_params(const hb_ek_t* ek, hb_params_t* out);
+
+/* Evaluator::nand (tfhe/src/bfhe/evaluate.rs:113-121): one gate, one launch. */
+int hb_nand(hb_ek_t* ek, const uint64_t* c0, const uint64_t* c1, uint64_t* out);
+/* Evaluator::{and, or, nor, xor, xnor, majority} (evaluate.rs:129-231); c2 only for majority. */
+int hb_gate(hb_ek_t* ek, int gate, const uint64_t* c0, const uint64_t* c1, const uint64_t* c2,
+            uint64_t* out);
+/* Evaluator::bootstrap (evaluate.rs:86-88 → :42-60) with an arbitrary LUT of N words. */
+int hb_bootstrap(hb_ek_t* ek, const uint64_t* c, const uint64_t* lut, uint64_t* out);
+/* Evaluator::not (evaluate.rs:98-104): linear, no bootstrap. */
+int hb_not(const hb_ek_t* ek, const uint64_t* c, uint64_t* out);
+
+/* Batched gates: `count` independent gates of one kind in a single launch (no
+ * reference counterpart; the reference runs gates one per thread, tfhe/examples/bfhe.rs:139-148).
+ * Host arrays of count*(n+1) words; c2s only for majority. */
+int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,
+                  const uint64_t* c2s, uint64_t* outs);
+/* Same on device-resident arrays; `stream` is a hipStream_t or NULL. Asynchronous. */
+int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_dev,
+                      const uint64_t* c1s_dev, const uint64_t* c2s_dev, uint64_t* outs_dev,
+                      void* stream);
+
+/* Standalone negacyclic NTT over `count` polynomials of N = 2^log_n words (device
+ * pointers, canonical in and out, in place). Forward output in the key layout's slot
+ * order. Replaces AbstractNTT::{transform_slice, inverse_transform_slice}
+ * (algebra/src/transformation/mod.rs:85,95). field: 0 BabyBear, 1 Goldilocks. */
+int hb_ntt_dev(int field, int inverse, uint32_t log_n, uint64_t* data_dev, size_t count, void* stream);
+
+/* External product of `count` RLWEs (2N words each, a then b, coefficient domain,
+ * canonical) with one NTT-domain RGSW in the canonical layout [2][len][2][N]
+ * (RLWE::mul_assign_ntt_rgsw, lattice/src/rlwe.rs:573-596). Device pointers. */
+int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t len,
+                            const uint64_t* rgsw_dev, uint64_t* rlwe_dev, size_t count,
+                            void* stream);
+
+/* Device memory helpers so callers without a HIP runtime of their own can stage data. */
+int hb_dev_alloc(size_t bytes, void** out);
+int hb_dev_free(void* p);
+int hb_memcpy_h2d(void* dst_dev, const void* src, size_t bytes);
+int hb_memcpy_d2h(void* dst, const void* src_dev, size_t bytes);
+int hb_sync(void);
+int hb_device_count(int* out);
+
+/* Average duration in ms of the last `hb_*_dev` kernel launch, measured with HIP events
+ * recorded on the stream the kernel ran on. */
+int hb_last_kernel_ms(double* out);
+
+/* Human-readable message for the last error on the calling thread. */
+const char* hb_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HB_H_ */
