@@ -1,0 +1,234 @@
+"""Benchmark: batched NAND gate bootstrapping on one MI355X.
+
+Workload (default): 4096 independent NAND gates with GOLDILOCKS_BINARY_128_BITS_PARAMETERS
+in one kernel launch, keys (BSK + KSK) resident in HBM, ciphertexts already in HBM when
+the timed region starts. One step = one such launch. Also reported in the same run:
+  plain   — the same kernel with one gate per launch (bootstraps/s and p50/p99 latency),
+  roofline — key + I/O bytes per launch over the kernel's event-timed duration vs 8 TB/s,
+  cpu_baseline — the CPU oracle (a restatement of the reference path) on the host cores.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+SETS = {"babybear": 0, "goldilocks": 1, "zama": 2}
+SET_NAMES = {0: "BABYBEAR_BINARY_128_BITS", 1: "GOLDILOCKS_BINARY_128_BITS",
+             2: "ZAMA_BINARY_128_BITS"}
+METRIC = "NAND bootstraps/sec, batched vs plain on 1 MI355X; p50/p99 gate latency"
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def modmuls_per_gate(P) -> int:
+    """Modular multiplications one bootstrap performs in this engine (all rounds active):
+    per round 2*br_len forward + 2 inverse NTTs of (N/2)log2(N) butterflies and 4*br_len*N
+    MAC products; key switching ks_len forward + 1 inverse NTT and 2*ks_len*N products."""
+    N, logn = P.N, P.N.bit_length() - 1
+    ntt = (N // 2) * logn
+    per_round = (2 * P.br_len + 2) * ntt + 4 * P.br_len * N
+    ks = (P.ks_len + 1) * ntt + 2 * P.ks_len * N
+    return P.n * per_round + ks
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--set", choices=list(SETS), default="goldilocks")
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--plain", type=int, default=8, help="gates timed on the plain path")
+    ap.add_argument("--check", type=int, default=16, help="gates compared with the oracle")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the engine has no multi-GPU path "
+              "(gates shard across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+
+    import hasteboots_amd as hb
+    from hasteboots_amd._lib import check, lib
+
+    if hb.device_count() < 1:
+        log("no GPU visible")
+        return 1
+    L = lib()
+    set_id = SETS[args.set]
+    P = hb.Parameters.of(set_id)
+    G, n1 = args.batch, P.n + 1
+
+    t = time.time()
+    sk = hb.KeyGen.generate_secret_key(P, seed=0)
+    bsk, ksk = sk.generate_keys_host()
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P, device=0)
+    log(f"keygen+upload {time.time() - t:.2f}s ({args.set})")
+
+    rng = np.random.default_rng([0, 3])
+    b0 = rng.integers(0, 2, size=G)
+    b1 = rng.integers(0, 2, size=G)
+    enc = hb.Encryptor(sk)
+    c0 = enc.encrypt_many(b0)
+    c1 = enc.encrypt_many(b1)
+
+    def dev(arr):
+        p = C.c_void_p()
+        check(L.hb_dev_alloc(arr.nbytes, C.byref(p)))
+        check(L.hb_memcpy_h2d(p, arr.ctypes.data, arr.nbytes))
+        return p
+
+    d0, d1 = dev(c0), dev(c1)
+    dout = dev(np.zeros_like(c0))
+
+    def launch(count, a=d0, b=d1, o=dout):
+        check(L.hb_gate_batch_dev(ev._h, 0, count, a, b, None, o, None))
+
+    for _ in range(args.warmup):
+        launch(G)
+        check(L.hb_sync())
+    log(f"warmup done ({args.warmup} launches)")
+
+    kernel_ms = []
+    check(L.hb_sync())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        launch(G)
+        kernel_ms.append(hb.last_kernel_ms())
+    check(L.hb_sync())
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = G * args.steps / wall
+    log(f"batched: {value:.1f} bootstraps/s, kernel {np.mean(kernel_ms):.1f} ms/launch")
+
+    # correctness of the measured launch
+    out = np.zeros_like(c0)
+    check(L.hb_memcpy_d2h(out.ctypes.data, dout, out.nbytes))
+    dec = hb.Decryptor(sk)
+    wrong = sum(dec.decrypt(hb.LWECiphertext(out[i])) != 1 - (int(b0[i]) & int(b1[i]))
+                for i in range(G))
+    oracle_ok = None
+    if args.check > 0:
+        import oracle
+        k = min(args.check, G)
+        ref = oracle.OracleKeys(set_id, bsk, ksk).gate_batch(
+            0, c0[:k], c1[:k], threads=min(16, os.cpu_count() or 1))
+        oracle_ok = bool(np.array_equal(ref, out[:k]))
+    log(f"decrypt mismatches {wrong}/{G}; oracle bit-exact on {args.check}: {oracle_ok}")
+
+    # plain path: one gate per launch
+    lat = []
+    for i in range(args.plain):
+        check(L.hb_sync())
+        s = time.perf_counter()
+        launch(1, C.c_void_p(d0.value + i * n1 * 8), C.c_void_p(d1.value + i * n1 * 8),
+               C.c_void_p(dout.value + i * n1 * 8))
+        check(L.hb_sync())
+        lat.append((time.perf_counter() - s) * 1e3)
+    plain = None
+    if lat:
+        plain_value = 1e3 / np.mean(lat)
+        plain = {"value": round(plain_value, 3), "unit": "bootstraps/s",
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3), "gates": args.plain}
+        log(f"plain: {plain_value:.2f} bootstraps/s, p50 {np.percentile(lat, 50):.1f} ms")
+
+    wb = 8 if P.is_goldilocks else 4
+    key_bytes = (bsk.size + ksk.size) * wb
+    io_bytes = G * 3 * n1 * 8
+    traffic_alg = key_bytes + io_bytes
+    kms = float(np.mean(kernel_ms))
+    achieved = traffic_alg / (kms * 1e-3)
+    mm = modmuls_per_gate(P)
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = min(16, os.cpu_count() or 1)
+        ok = oracle.OracleKeys(set_id, bsk, ksk)
+        s = time.perf_counter()
+        ok.gate_batch(0, c0[:1], c1[:1], threads=1)
+        single = time.perf_counter() - s
+        per_thread = max(1, int(round(args.cpu_seconds / max(single, 1e-3))))
+        per_thread = min(per_thread, max(1, G // threads))
+        S = threads * per_thread
+        s = time.perf_counter()
+        ok.gate_batch(0, c0[:S], c1[:S], threads=threads)
+        el = time.perf_counter() - s
+        cpu = {"value": round(S / el, 3), "unit": "bootstraps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"{S} NAND gates ({args.set}) on {threads} threads, one gate per "
+                         f"thread task; single-thread latency {single * 1e3:.1f} ms",
+               "single_thread_ms": round(single * 1e3, 2), "cpu_model": cpu_model(),
+               "nproc": os.cpu_count()}
+        log(f"cpu baseline: {cpu['value']} bootstraps/s on {threads} threads, "
+            f"single {single * 1e3:.0f} ms")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "bootstraps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64" if P.is_goldilocks else "u32",
+        "data": "synthetic (seeded keys and uniform random input bits)",
+        "config": {"workload": f"batched NAND bootstrap, {SET_NAMES[set_id]}, "
+                               f"{G} gates per launch, keys resident in HBM",
+                   "parameter_set": SET_NAMES[set_id], "gates_per_launch": G,
+                   "n": P.n, "N": P.N, "br_bits": P.br_bits, "br_len": P.br_len,
+                   "ks_bits": P.ks_bits, "ks_len": P.ks_len},
+        "batched_vs_plain": round(value / plain["value"], 1) if plain else None,
+        "plain": plain,
+        "batch_latency_ms": {"p50": round(float(np.percentile(kernel_ms, 50)), 3),
+                             "p99": round(float(np.percentile(kernel_ms, 99)), 3)},
+        "kernel_ms_per_launch": round(kms, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 3),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "algorithmic_bytes_per_launch": traffic_alg},
+        "valu": {"modmul_per_gate": mm,
+                 "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1)},
+        "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
+                    "oracle_gates": args.check},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for p in (d0, d1, dout):
+        L.hb_dev_free(p)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -11,7 +11,8 @@ import re
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
-LIB_PATH = PKG / "libhasteboots.so"
+# HB_LIB_PATH selects another build of the same library (e.g. for A/B timing).
+LIB_PATH = Path(os.environ.get("HB_LIB_PATH", PKG / "libhasteboots.so"))
 HEADER = PKG.parent / "include" / "hb.h"
 
 u64p = C.POINTER(C.c_uint64)
