@@ -18,4 +18,5 @@ from .tfhe import (  # noqa: F401
     SecretKeyPack,
     device_count,
     last_kernel_ms,
+    mixed_batch,
 )

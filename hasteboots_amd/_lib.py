@@ -30,6 +30,12 @@ class HbParams(C.Structure):
     ]
 
 
+class HbBatchPart(C.Structure):
+    _fields_ = [("ek", C.c_void_p), ("gate", C.c_int32), ("pad_", C.c_uint32),
+                ("count", C.c_uint64), ("c0s", C.c_void_p), ("c1s", C.c_void_p),
+                ("c2s", C.c_void_p), ("outs", C.c_void_p)]
+
+
 class HbError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"hb error {code}: {msg}")
@@ -57,6 +63,8 @@ _PROTOS = {
     "hb_gate_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]),
     "hb_gate_batch_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hb_mixed_batch": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t]),
+    "hb_mixed_batch_dev": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t, C.c_void_p]),
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "hb_external_product_dev": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),

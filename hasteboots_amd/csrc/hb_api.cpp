@@ -2,6 +2,7 @@
 // device layout, and launches of the HIP kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,13 @@ struct Engine {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   DeviceTables tables;
+  // mixed-parameter launches: accumulator scratch, part descriptors, host staging
+  void* mixed_acc = nullptr;
+  size_t mixed_acc_bytes = 0;
+  BootArgs* mixed_parts = nullptr;
+  size_t mixed_parts_cap = 0;
+  uint64_t* mixed_io = nullptr;
+  size_t mixed_io_words = 0;
 };
 
 Engine& engine() {
@@ -208,12 +216,11 @@ void lut_params(const ParamSet& p, int gate, BootArgs& a) {
   a.lut_hi = hi;
 }
 
-int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
-                 const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {
+BootArgs make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* c0,
+                   const uint64_t* c1, const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out,
+                   void* acc) {
   const ParamSet& p = ek->p;
   Engine& E = engine();
-  int rc = ensure_acc(ek, count);
-  if (rc) return rc;
   BootArgs a{};
   a.bsk = ek->bsk_dev;
   a.ksk = ek->ksk_dev;
@@ -225,7 +232,7 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   a.c2 = c2;
   a.lut = lut_dev;
   a.out = out;
-  a.acc = ek->acc_dev;
+  a.acc = acc;
   a.qf = (double)p.q;
   a.q8 = p.q >> 3;
   a.n = p.n;
@@ -235,7 +242,19 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   a.ks_bits = p.ks_bits;
   a.count = (uint32_t)count;
   a.gate = gate;
+  a.is_goldilocks = p.is_goldilocks;
+  a.first_block = 0;
   lut_params(p, gate, a);
+  return a;
+}
+
+int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
+                 const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {
+  const ParamSet& p = ek->p;
+  Engine& E = engine();
+  int rc = ensure_acc(ek, count);
+  if (rc) return rc;
+  BootArgs a = make_args(ek, gate, count, c0, c1, c2, lut_dev, out, ek->acc_dev);
   HIP_TRY(hipEventRecord(E.ev0, s));
   HIP_TRY(launch_bootstrap(a, p.is_goldilocks, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
@@ -426,6 +445,113 @@ int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, cons
   rc = launch_gates(ek, gate, count, d0, d1, c2s ? d2 : nullptr, nullptr, dout, E.stream);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream));
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return record_ms(E.stream);
+}
+
+int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream) {
+  if (!parts && nparts) return fail(HB_ERR_ARG, "null parts");
+  Engine& E = engine();
+  std::vector<size_t> order;
+  uint64_t total = 0;
+  for (size_t i = 0; i < nparts; i++) {
+    const hb_batch_part_t& q = parts[i];
+    if (!q.ek) return fail(HB_ERR_ARG, "part %zu: null ek", i);
+    if (q.gate < 0 || q.gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "part %zu: bad gate", i);
+    if (q.count == 0) continue;
+    if (!q.c0s || !q.c1s || !q.outs) return fail(HB_ERR_ARG, "part %zu: null buffer", i);
+    if (q.gate == GATE_MAJORITY && !q.c2s) return fail(HB_ERR_ARG, "part %zu: majority needs c2", i);
+    total += q.count;
+    order.push_back(i);
+  }
+  if (total == 0) return HB_OK;
+  if (total > 0x7FFFFFFF) return fail(HB_ERR_ARG, "mixed batch too large");
+  // Heaviest parameter sets first so the light gates fill the tail of the grid.
+  auto cost = [&](size_t i) {
+    const ParamSet& p = parts[i].ek->p;
+    return (double)(p.br_len + 2) * (p.is_goldilocks ? 2.0 : 1.0);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
+  std::lock_guard<std::mutex> g(E.mu);
+  const size_t per_gate = (size_t)2 * 1024 * 8;
+  if (E.mixed_acc_bytes < total * per_gate) {
+    if (E.mixed_acc) HIP_TRY(hipFree(E.mixed_acc));
+    E.mixed_acc = nullptr;
+    E.mixed_acc_bytes = 0;
+    HIP_TRY(hipMalloc(&E.mixed_acc, total * per_gate));
+    E.mixed_acc_bytes = total * per_gate;
+  }
+  if (E.mixed_parts_cap < order.size()) {
+    if (E.mixed_parts) HIP_TRY(hipFree(E.mixed_parts));
+    E.mixed_parts = nullptr;
+    E.mixed_parts_cap = 0;
+    HIP_TRY(hipMalloc((void**)&E.mixed_parts, order.size() * sizeof(BootArgs)));
+    E.mixed_parts_cap = order.size();
+  }
+  std::vector<BootArgs> args;
+  uint32_t first = 0;
+  for (size_t i : order) {
+    const hb_batch_part_t& q = parts[i];
+    if (q.ek->p.N != 1024) return fail(HB_ERR_ARG, "part %zu: unsupported ring dimension", i);
+    BootArgs a = make_args(q.ek, q.gate, q.count, q.c0s, q.c1s, q.c2s, nullptr, q.outs,
+                           (char*)E.mixed_acc + (size_t)first * per_gate);
+    a.first_block = first;
+    first += (uint32_t)q.count;
+    args.push_back(a);
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  HIP_TRY(hipMemcpyAsync(E.mixed_parts, args.data(), args.size() * sizeof(BootArgs),
+                         hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
+  HIP_TRY(hipEventRecord(E.ev0, s));
+  HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
+  HIP_TRY(hipEventRecord(E.ev1, s));
+  return HB_OK;
+}
+
+int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts) {
+  if (!parts && nparts) return fail(HB_ERR_ARG, "null parts");
+  Engine& E = engine();
+  uint64_t words = 0;
+  for (size_t i = 0; i < nparts; i++) {
+    if (!parts[i].ek) return fail(HB_ERR_ARG, "part %zu: null ek", i);
+    words += parts[i].count * (uint64_t)(parts[i].ek->p.n + 1) * 4;
+  }
+  std::vector<hb_batch_part_t> dev(parts, parts + nparts);
+  {
+    std::lock_guard<std::mutex> g(E.mu);
+    if (E.mixed_io_words < words) {
+      if (E.mixed_io) HIP_TRY(hipFree(E.mixed_io));
+      E.mixed_io = nullptr;
+      E.mixed_io_words = 0;
+      HIP_TRY(hipMalloc((void**)&E.mixed_io, words * 8));
+      E.mixed_io_words = words;
+    }
+    uint64_t off = 0;
+    for (size_t i = 0; i < nparts; i++) {
+      const hb_batch_part_t& q = parts[i];
+      const size_t bytes = q.count * (size_t)(q.ek->p.n + 1) * 8;
+      if (q.count == 0) continue;
+      if (!q.c0s || !q.c1s || !q.outs) return fail(HB_ERR_ARG, "part %zu: null buffer", i);
+      uint64_t* base = E.mixed_io + off;
+      const size_t w = bytes / 8;
+      HIP_TRY(hipMemcpyAsync(base, q.c0s, bytes, hipMemcpyHostToDevice, E.stream));
+      HIP_TRY(hipMemcpyAsync(base + w, q.c1s, bytes, hipMemcpyHostToDevice, E.stream));
+      if (q.c2s) HIP_TRY(hipMemcpyAsync(base + 2 * w, q.c2s, bytes, hipMemcpyHostToDevice, E.stream));
+      dev[i].c0s = base;
+      dev[i].c1s = base + w;
+      dev[i].c2s = q.c2s ? base + 2 * w : nullptr;
+      dev[i].outs = base + 3 * w;
+      off += 4 * w;
+    }
+  }
+  int rc = hb_mixed_batch_dev(dev.data(), nparts, nullptr);
+  if (rc) return rc;
+  for (size_t i = 0; i < nparts; i++) {
+    if (parts[i].count == 0) continue;
+    const size_t bytes = parts[i].count * (size_t)(parts[i].ek->p.n + 1) * 8;
+    HIP_TRY(hipMemcpyAsync(parts[i].outs, dev[i].outs, bytes, hipMemcpyDeviceToHost, E.stream));
+  }
   HIP_TRY(hipStreamSynchronize(E.stream));
   return record_ms(E.stream);
 }

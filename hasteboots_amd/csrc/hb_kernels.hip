@@ -301,6 +301,23 @@ __global__ __launch_bounds__(64) void bootstrap_kernel(BootArgs A) {
   bootstrap_gate<F>(A, blockIdx.x, lds, ms);
 }
 
+// Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
+// Each part carries its own keys, twiddles, gate kind and I/O pointers, so gates of
+// BabyBear, Goldilocks and Zama-Goldilocks run side by side in one grid.
+__global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ parts, int nparts) {
+  __shared__ u64 lds[1024 + 64];
+  __shared__ unsigned short ms[1024];
+  const u32 b = blockIdx.x;
+  int p = 0;
+  while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
+  const BootArgs A = parts[p];
+  const int g = (int)(b - A.first_block);
+  if (A.is_goldilocks)
+    bootstrap_gate<Gl>(A, g, lds, ms);
+  else
+    bootstrap_gate<Bb>(A, g, reinterpret_cast<u32*>(lds), ms);
+}
+
 // ----------------------------------------------------------------------------
 // Microbench: negacyclic NTT over `count` polynomials, one wave per polynomial,
 // 4 waves per workgroup. Forward: natural order in, slot order out. Inverse: slot
@@ -394,6 +411,11 @@ hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s)
     hipLaunchKernelGGL(bootstrap_kernel<Gl>, grid, block, 0, s, a);
   else
     hipLaunchKernelGGL(bootstrap_kernel<Bb>, grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {
+  hipLaunchKernelGGL(mixed_kernel, dim3(total_gates), dim3(64), 0, s, parts_dev, nparts);
   return hipGetLastError();
 }
 

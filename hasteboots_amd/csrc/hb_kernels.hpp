@@ -36,9 +36,14 @@ struct BootArgs {
   uint32_t n, br_len, br_bits, ks_len, ks_bits;
   uint32_t count;
   int gate;
+  int is_goldilocks;
+  uint32_t first_block;  // mixed launches: first workgroup index of this part
 };
 
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s);
+// One launch over `nparts` parts (device array of BootArgs, parts in block order).
+hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
+                        hipStream_t s);
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
                       const void* tw, uint64_t ninv, hipStream_t s);
 hipError_t launch_extprod(int is_goldilocks, uint64_t* rlwe, size_t count, const void* rgsw,

@@ -279,6 +279,35 @@ class Evaluator:
         return self.gate_batch("nand", c0s, c1s)
 
 
+def mixed_batch(parts) -> list[np.ndarray]:
+    """Bootstrap gates of several evaluation keys (any parameter sets) in ONE launch.
+
+    parts: iterable of (evaluator, gate_name, c0s, c1s[, c2s]) with arrays of shape
+    (count, n+1). Returns the output arrays in the same order."""
+    from ._lib import HbBatchPart
+    parts = list(parts)
+    descs = (HbBatchPart * max(1, len(parts)))()
+    keep, outs = [], []
+    for i, part in enumerate(parts):
+        ev, gate, c0s, c1s = part[:4]
+        c2s = part[4] if len(part) > 4 else None
+        c0s = np.ascontiguousarray(c0s, np.uint64)
+        c1s = np.ascontiguousarray(c1s, np.uint64)
+        c2s = None if c2s is None else np.ascontiguousarray(c2s, np.uint64)
+        out = np.zeros_like(c0s)
+        keep += [c0s, c1s, c2s]
+        outs.append(out)
+        d = descs[i]
+        d.ek = ev._h
+        d.gate = GATES[gate]
+        d.count = c0s.shape[0]
+        d.c0s, d.c1s = c0s.ctypes.data, c1s.ctypes.data
+        d.c2s = c2s.ctypes.data if c2s is not None else None
+        d.outs = out.ctypes.data
+    check(lib().hb_mixed_batch(descs, len(parts)))
+    return outs
+
+
 def last_kernel_ms() -> float:
     v = C.c_double()
     check(lib().hb_last_kernel_ms(C.byref(v)))

@@ -125,6 +125,25 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_d
                       const uint64_t* c1s_dev, const uint64_t* c2s_dev, uint64_t* outs_dev,
                       void* stream);
 
+/* Mixed-parameter batch: several parts, each with its own evaluation key (any of the
+ * three parameter sets) and gate kind, bootstrapped in ONE kernel launch. No reference
+ * counterpart (the reference evaluates one gate per call). Heavier parameter sets are
+ * scheduled first. All keys must live on the engine's device. */
+typedef struct {
+  hb_ek_t* ek;
+  int32_t gate;
+  uint32_t pad_;
+  uint64_t count;
+  const uint64_t* c0s;
+  const uint64_t* c1s;
+  const uint64_t* c2s; /* majority only */
+  uint64_t* outs;
+} hb_batch_part_t;
+/* Host pointers; returns when outputs are on the host. */
+int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts);
+/* Device pointers; asynchronous on `stream` (NULL = the engine's stream). */
+int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream);
+
 /* Standalone negacyclic NTT over `count` polynomials of N = 2^log_n words (device
  * pointers, canonical in and out, in place). Forward output in the key layout's slot
  * order. Replaces AbstractNTT::{transform_slice, inverse_transform_slice}

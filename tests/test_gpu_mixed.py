@@ -1,0 +1,86 @@
+"""Mixed-parameter batches: gates of BabyBear, Goldilocks and Zama-Goldilocks keys in one
+kernel launch, each output bit-exact with the oracle for its own parameter set."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import stream
+
+pytestmark = pytest.mark.gpu
+
+TRUTH = {"nand": lambda a, b, c: 1 - (a & b), "and": lambda a, b, c: a & b,
+         "or": lambda a, b, c: a | b, "xor": lambda a, b, c: a ^ b,
+         "majority": lambda a, b, c: int(a + b + c >= 2)}
+GATE_ID = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def evaluators(hb, keyset):
+    evs = {}
+    for s in (0, 1, 2):
+        P, sk, bsk, ksk = keyset(s)
+        evs[s] = (P, sk, bsk, ksk, hb.Evaluator(keys=(bsk, ksk), params=P))
+    return evs
+
+
+def test_mixed_small_all_gates_bit_exact(hb, orc, evaluators):
+    say("mixed small")
+    rng = stream(40, "mixed-small")
+    parts, meta = [], []
+    for s, gate, count in [(0, "nand", 20), (1, "xor", 10), (2, "and", 6), (0, "majority", 8),
+                           (1, "nand", 0), (2, "or", 5)]:
+        P, sk, bsk, ksk, ev = evaluators[s]
+        enc = hb.Encryptor(sk)
+        bits = rng.integers(0, 2, size=(3, count))
+        cs = [enc.encrypt_many(b) if count else np.zeros((0, P.n + 1), np.uint64) for b in bits]
+        parts.append((ev, gate, cs[0], cs[1], cs[2] if gate == "majority" else None))
+        meta.append((s, gate, bits, cs))
+    outs = hb.mixed_batch(parts)
+    for (s, gate, bits, cs), out in zip(meta, outs):
+        if out.shape[0] == 0:
+            continue
+        P, sk, bsk, ksk, ev = evaluators[s]
+        ok = orc.OracleKeys(s, bsk, ksk)
+        ref = ok.gate_batch(GATE_ID[gate], cs[0], cs[1], cs[2] if gate == "majority" else None,
+                            threads=16)
+        assert np.array_equal(out, ref), (s, gate)
+        dec = hb.Decryptor(sk)
+        for i in range(out.shape[0]):
+            expect = TRUTH[gate](*(int(bits[k][i]) for k in range(3)))
+            assert dec.decrypt(hb.LWECiphertext(out[i])) == expect
+
+
+def test_mixed_geometric_counts_config(hb, orc, evaluators):
+    """Per-set counts ~ Geometric(mean 300) truncated at 2048, uniform bits, one launch.
+    Every output decrypts to NAND; a seeded sample per set is compared word for word with
+    the oracle, and a few gates evaluated alone equal their results inside the mix."""
+    rng = stream(41, "mixed-config")
+    counts = [int(min(2048, rng.geometric(1 / 300))) for _ in range(3)]
+    say(f"mixed geometric counts {counts}")
+    parts, data = [], []
+    for s, count in zip((0, 1, 2), counts):
+        P, sk, bsk, ksk, ev = evaluators[s]
+        enc = hb.Encryptor(sk)
+        b0 = rng.integers(0, 2, size=count)
+        b1 = rng.integers(0, 2, size=count)
+        c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+        parts.append((ev, "nand", c0, c1))
+        data.append((b0, b1, c0, c1))
+    outs = hb.mixed_batch(parts)
+    for s, out, (b0, b1, c0, c1) in zip((0, 1, 2), outs, data):
+        P, sk, bsk, ksk, ev = evaluators[s]
+        dec = hb.Decryptor(sk)
+        bad = sum(dec.decrypt(hb.LWECiphertext(out[i])) != 1 - (int(b0[i]) & int(b1[i]))
+                  for i in range(out.shape[0]))
+        assert bad == 0, (s, bad)
+        pick = np.unique(rng.integers(0, out.shape[0], size=12))
+        ref = orc.OracleKeys(s, bsk, ksk).gate_batch(0, c0[pick], c1[pick], threads=16)
+        assert np.array_equal(out[pick], ref), s
+        alone = ev.nand(hb.LWECiphertext(c0[pick[0]]), hb.LWECiphertext(c1[pick[0]]))
+        assert np.array_equal(alone.words, out[pick[0]])
