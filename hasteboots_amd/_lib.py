@@ -36,6 +36,19 @@ class HbBatchPart(C.Structure):
                 ("c2s", C.c_void_p), ("outs", C.c_void_p)]
 
 
+class HbCircuit(C.Structure):
+    _fields_ = [("n_inputs", C.c_uint32), ("n_gates", C.c_uint32), ("n_outputs", C.c_uint32),
+                ("pad_", C.c_uint32), ("in0", C.c_void_p), ("in1", C.c_void_p),
+                ("kind", C.c_void_p), ("outputs", C.c_void_p)]
+
+
+class HbServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64),
+                ("gates_done", C.c_uint64), ("launches", C.c_uint64), ("queued", C.c_uint64),
+                ("active", C.c_uint64), ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64),
+                ("busy_ms", C.c_double)]
+
+
 class HbError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"hb error {code}: {msg}")
@@ -63,6 +76,13 @@ _PROTOS = {
     "hb_gate_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]),
     "hb_gate_batch_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hb_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "hb_server_destroy": (C.c_int, [C.c_void_p]),
+    "hb_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(HbCircuit), u64p,
+                                   C.POINTER(C.c_uint64)]),
+    "hb_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
+    "hb_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.POINTER(C.c_double)]),
+    "hb_server_stats": (C.c_int, [C.c_void_p, C.POINTER(HbServerStats)]),
     "hb_mixed_batch": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t]),
     "hb_mixed_batch_dev": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t, C.c_void_p]),
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),

@@ -13,14 +13,17 @@
 
 #include "../../include/hb.h"
 #include "hb_field.hpp"
+#include "hb_internal.hpp"
 #include "hb_kernels.hpp"
 #include "hb_keys.hpp"
 
 using namespace hb;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace hb {
 
 int fail(int code, const char* fmt, ...) {
   char buf[512];
@@ -32,40 +35,14 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(HB_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
-  } while (0)
-
-// Per-device constant tables: twiddles for log N = 10, 11 in device word form.
-struct DeviceTables {
-  bool ready = false;
-  void* tw[2][2][2] = {};  // [field][logN-10][inverse]
-  uint64_t ninv[2][2] = {};  // device-form N^-1 for the microbench inverse
-};
-
-struct Engine {
-  std::mutex mu;
-  int device = -1;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
-  DeviceTables tables;
-  // mixed-parameter launches: accumulator scratch, part descriptors, host staging
-  void* mixed_acc = nullptr;
-  size_t mixed_acc_bytes = 0;
-  BootArgs* mixed_parts = nullptr;
-  size_t mixed_parts_cap = 0;
-  uint64_t* mixed_io = nullptr;
-  size_t mixed_io_words = 0;
-};
-
 Engine& engine() {
   static Engine e;
   return e;
 }
+
+}  // namespace hb
+
+namespace {
 
 uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
   // scale: a canonical multiplier folded into the constant (e.g. N^-1).
@@ -74,7 +51,9 @@ uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
   return Bb::to_mont(x);
 }
 
-int ensure_engine(int device) {
+}  // namespace
+
+int hb::ensure_engine(int device) {
   Engine& E = engine();
   if (E.device >= 0) {
     if (device != E.device) return fail(HB_ERR_ARG, "engine already bound to device %d", E.device);
@@ -117,6 +96,8 @@ int ensure_engine(int device) {
   return HB_OK;
 }
 
+namespace {
+
 uint32_t ilog2u(uint32_t x) {
   uint32_t r = 0;
   while ((1u << r) < x) r++;
@@ -124,24 +105,6 @@ uint32_t ilog2u(uint32_t x) {
 }
 
 }  // namespace
-
-struct hb_sk {
-  SecretKey sk;
-};
-
-struct hb_ek {
-  ParamSet p;
-  std::vector<uint64_t> bsk_host, ksk_host;  // canonical layout (kept for export)
-  void* bsk_dev = nullptr;
-  void* ksk_dev = nullptr;
-  void* lut_dev = nullptr;  // N words staging for hb_bootstrap
-  // staging for host-pointer calls and the accumulator scratch
-  std::mutex mu;
-  uint64_t* io_dev = nullptr;
-  size_t io_cap = 0;  // gates
-  void* acc_dev = nullptr;
-  size_t acc_cap = 0;  // gates
-};
 
 namespace {
 
@@ -216,9 +179,11 @@ void lut_params(const ParamSet& p, int gate, BootArgs& a) {
   a.lut_hi = hi;
 }
 
-BootArgs make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* c0,
-                   const uint64_t* c1, const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out,
-                   void* acc) {
+}  // namespace
+
+BootArgs hb::make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* c0,
+                       const uint64_t* c1, const uint64_t* c2, const uint64_t* lut_dev,
+                       uint64_t* out, void* acc) {
   const ParamSet& p = ek->p;
   Engine& E = engine();
   BootArgs a{};
@@ -247,6 +212,8 @@ BootArgs make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* c0,
   lut_params(p, gate, a);
   return a;
 }
+
+namespace {
 
 int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
                  const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {

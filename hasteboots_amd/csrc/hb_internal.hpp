@@ -1,0 +1,77 @@
+// Engine internals shared by the C-ABI translation units (hb_api.cpp, hb_server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hb.h"
+#include "hb_kernels.hpp"
+#include "hb_keys.hpp"
+#include "hb_params.hpp"
+
+namespace hb {
+
+// Records a message for hb_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::hb::fail(HB_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+// Per-device constant tables: twiddles for log N = 10, 11 in device word form.
+struct DeviceTables {
+  bool ready = false;
+  void* tw[2][2][2] = {};    // [field][logN-10][inverse]
+  uint64_t ninv[2][2] = {};  // device-form N^-1 for the microbench inverse
+};
+
+// The process-wide engine: one device, its default stream, twiddle tables and the
+// scratch of mixed-parameter launches.
+struct Engine {
+  std::mutex mu;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  DeviceTables tables;
+  void* mixed_acc = nullptr;
+  size_t mixed_acc_bytes = 0;
+  BootArgs* mixed_parts = nullptr;
+  size_t mixed_parts_cap = 0;
+  uint64_t* mixed_io = nullptr;
+  size_t mixed_io_words = 0;
+};
+
+Engine& engine();
+int ensure_engine(int device);
+
+}  // namespace hb
+
+struct hb_sk {
+  hb::SecretKey sk;
+};
+
+struct hb_ek {
+  hb::ParamSet p;
+  std::vector<uint64_t> bsk_host, ksk_host;  // canonical layout (kept for export)
+  void* bsk_dev = nullptr;
+  void* ksk_dev = nullptr;
+  void* lut_dev = nullptr;  // N words staging for hb_bootstrap
+  std::mutex mu;            // guards the staging buffers below
+  uint64_t* io_dev = nullptr;
+  size_t io_cap = 0;  // gates
+  void* acc_dev = nullptr;
+  size_t acc_cap = 0;  // gates
+};
+
+namespace hb {
+// Kernel arguments for `count` gates of `ek` (see BootArgs).
+BootArgs make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* c0,
+                   const uint64_t* c1, const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out,
+                   void* acc);
+}  // namespace hb

@@ -166,9 +166,12 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   W* accB = accA + N;
 
   // 1. gate combination + modulus switch of all n+1 coefficients.
-  const u64* c0 = A.c0 + (size_t)g * n1;
-  const u64* c1 = A.c1 ? A.c1 + (size_t)g * n1 : nullptr;
-  const u64* c2 = A.c2 ? A.c2 + (size_t)g * n1 : nullptr;
+  // Either dense arrays (gate g at row g) or, for circuits, wire indices into one
+  // device-resident wire buffer: idx[3g..3g+2] = (input 0, input 1, output).
+  const u32* wi = A.idx ? A.idx + 3 * (size_t)g : nullptr;
+  const u64* c0 = A.c0 + (size_t)(wi ? wi[0] : g) * n1;
+  const u64* c1 = wi ? A.c0 + (size_t)wi[1] * n1 : (A.c1 ? A.c1 + (size_t)g * n1 : nullptr);
+  const u64* c2 = (!wi && A.c2) ? A.c2 + (size_t)g * n1 : nullptr;
   for (u32 k = lane; k < n1; k += 64) {
     W a = (W)c0[k];
     W b = c1 ? (W)c1[k] : (W)0;
@@ -279,7 +282,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   const W bout = F::canon(DF<F>::add(F::canon(b0), s));
   ntt_inv_wave<F, E>(pa, lds, itw, lane);
   // extract_partial_lwe_locally (rlwe.rs:393-402): a' = [A0, -A_{N-1}, ..., -A_1][..n].
-  u64* out = A.out + (size_t)g * n1;
+  u64* out = A.out + (size_t)(wi ? wi[2] : g) * n1;
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int p = lane + 64 * e;

@@ -1,0 +1,464 @@
+// Serving loop: continuous batching of boolean circuits from many clients.
+//
+// Each request is a gate circuit (a DAG over wires) encrypted under one client's key.
+// At admission the request's wire buffer and gate table are placed in a device arena
+// whose size is the server's HBM budget; a request that does not fit waits in FIFO order
+// until completed requests release their space. A worker thread keeps up to two kernel
+// launches in flight (two streams); every launch gathers, from all admitted requests that
+// are not already running, the gates whose inputs are ready (the next layer), up to the
+// per-launch gate capacity. Gates read their inputs from and write their outputs to the
+// request's wires in HBM, so circuit state never leaves the device between layers.
+// There is no reference counterpart: the reference evaluates gates one call at a time.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+
+#include "hb_internal.hpp"
+
+using namespace hb;
+
+namespace {
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+// First-fit allocator over one device arena (offsets in bytes, 256-byte granules).
+class Arena {
+ public:
+  void init(size_t bytes) {
+    size_ = bytes;
+    free_.clear();
+    free_[0] = bytes;
+  }
+  bool alloc(size_t bytes, size_t* off) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        *off = it->first;
+        size_t rest = it->second - bytes, start = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[start] = rest;
+        used_ += bytes;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  size_t size() const { return size_; }
+  size_t used() const { return used_; }
+
+ private:
+  std::map<size_t, size_t> free_;
+  size_t size_ = 0, used_ = 0;
+};
+
+struct Request {
+  uint64_t id = 0;
+  hb_ek* ek = nullptr;
+  uint32_t n1 = 0, n_inputs = 0, n_gates = 0;
+  std::vector<uint32_t> outputs;
+  std::vector<uint32_t> idx;        // (in0, in1, out) per gate, in schedule order
+  std::vector<uint8_t> kind;        // gate kind per scheduled gate
+  std::vector<uint32_t> run_end;    // end of each run of one layer and one kind
+  std::vector<uint32_t> run_layer;  // layer of each run
+  std::vector<uint64_t> inputs;     // host copy until admission
+  size_t off = 0, bytes = 0, idx_off = 0;
+  uint32_t next = 0;       // first gate not yet launched
+  uint32_t completed = 0;  // gates finished
+  bool in_flight = false, finished = false;
+  std::vector<uint64_t> out;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+struct Slot {
+  hipStream_t s = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  BootArgs* parts_dev = nullptr;
+  BootArgs* parts_host = nullptr;  // pinned
+  void* acc = nullptr;
+  bool busy = false;
+  std::vector<std::pair<Request*, uint32_t>> work;
+  uint32_t gates = 0;
+  uint64_t seq = 0;  // launch order
+};
+
+}  // namespace
+
+struct hb_server {
+  int device = 0;
+  uint32_t max_gates = 0;
+  Arena arena;
+  char* arena_base = nullptr;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  bool stop = false;
+  std::thread worker;
+  uint64_t next_id = 1;
+  std::deque<Request*> queue;
+  std::vector<Request*> active;
+  std::map<uint64_t, std::unique_ptr<Request>> reqs;
+  Slot slots[2];
+  uint64_t launch_seq = 0;
+  hipStream_t copy = nullptr;
+  hb_server_stats_t st{};
+  std::string error;
+};
+
+namespace {
+
+constexpr size_t kAccPerGate = (size_t)2 * 1024 * 8;
+constexpr uint32_t kMaxParts = 4096;
+
+int admit(hb_server* S) {
+  while (!S->queue.empty()) {
+    Request* r = S->queue.front();
+    size_t off;
+    if (!S->arena.alloc(r->bytes, &off)) break;
+    r->off = off;
+    char* base = S->arena_base + off;
+    const size_t wires_bytes = (size_t)(r->n_inputs + r->n_gates) * r->n1 * 8;
+    r->idx_off = (wires_bytes + 255) & ~(size_t)255;
+    HIP_TRY(hipMemcpyAsync(base, r->inputs.data(), r->inputs.size() * 8, hipMemcpyHostToDevice,
+                           S->copy));
+    HIP_TRY(hipMemcpyAsync(base + r->idx_off, r->idx.data(), r->idx.size() * 4,
+                           hipMemcpyHostToDevice, S->copy));
+    HIP_TRY(hipStreamSynchronize(S->copy));
+    std::vector<uint64_t>().swap(r->inputs);
+    r->t_admit = now_ms();
+    S->active.push_back(r);
+    S->queue.pop_front();
+    S->st.admitted++;
+  }
+  S->st.bytes_in_use = S->arena.used();
+  S->st.queued = S->queue.size();
+  return HB_OK;
+}
+
+int finish(hb_server* S, Request* r) {
+  const char* base = S->arena_base + r->off;
+  r->out.resize((size_t)r->outputs.size() * r->n1);
+  for (size_t k = 0; k < r->outputs.size(); k++)
+    HIP_TRY(hipMemcpyAsync(r->out.data() + k * r->n1, base + (size_t)r->outputs[k] * r->n1 * 8,
+                           (size_t)r->n1 * 8, hipMemcpyDeviceToHost, S->copy));
+  HIP_TRY(hipStreamSynchronize(S->copy));
+  S->arena.release(r->off, r->bytes);
+  r->finished = true;
+  r->t_done = now_ms();
+  S->active.erase(std::find(S->active.begin(), S->active.end(), r));
+  S->st.completed++;
+  S->st.bytes_in_use = S->arena.used();
+  return HB_OK;
+}
+
+int complete(hb_server* S, Slot& sl) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) S->st.busy_ms += ms;
+  S->st.gates_done += sl.gates;
+  for (auto& w : sl.work) {
+    Request* r = w.first;
+    r->completed += w.second;
+    r->in_flight = false;
+    if (r->completed == r->n_gates) {
+      int rc = finish(S, r);
+      if (rc) return rc;
+    }
+  }
+  sl.work.clear();
+  sl.gates = 0;
+  sl.busy = false;
+  return HB_OK;
+}
+
+// Gathers the next ready run of each idle request into a launch on slot `sl`.
+int launch(hb_server* S, Slot& sl) {
+  uint32_t cap = S->max_gates, used = 0, nparts = 0;
+  for (Request* r : S->active) {
+    if (r->in_flight || r->next >= r->n_gates || used >= cap) continue;
+    // consecutive runs (one kind each) of the current layer, up to the capacity; the
+    // next layer waits for these outputs
+    size_t ri = std::upper_bound(r->run_end.begin(), r->run_end.end(), r->next) - r->run_end.begin();
+    const uint32_t layer = r->run_layer[ri];
+    uint32_t pos = r->next;
+    uint64_t* wires = reinterpret_cast<uint64_t*>(S->arena_base + r->off);
+    const uint32_t* idx = reinterpret_cast<const uint32_t*>(S->arena_base + r->off + r->idx_off);
+    while (ri < r->run_end.size() && r->run_layer[ri] == layer && used < cap && nparts < kMaxParts) {
+      const uint32_t cnt = std::min(r->run_end[ri] - pos, cap - used);
+      BootArgs a = make_args(r->ek, r->kind[pos], cnt, wires, nullptr, nullptr, nullptr, wires,
+                             (char*)sl.acc + (size_t)used * kAccPerGate);
+      a.idx = idx + 3 * (size_t)pos;
+      a.first_block = used;
+      sl.parts_host[nparts++] = a;
+      used += cnt;
+      pos += cnt;
+      if (pos < r->run_end[ri]) break;  // capacity reached inside the run
+      ri++;
+    }
+    const uint32_t take = pos - r->next;
+    if (take == 0) continue;
+    r->next += take;
+    r->in_flight = true;
+    sl.work.emplace_back(r, take);
+  }
+  if (used == 0) return HB_OK;
+  HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
+                         hipMemcpyHostToDevice, sl.s));
+  HIP_TRY(hipEventRecord(sl.ev0, sl.s));
+  HIP_TRY(launch_mixed(sl.parts_dev, (int)nparts, used, sl.s));
+  HIP_TRY(hipEventRecord(sl.ev1, sl.s));
+  sl.busy = true;
+  sl.gates = used;
+  sl.seq = ++S->launch_seq;
+  S->st.launches++;
+  return HB_OK;
+}
+
+}  // namespace
+
+namespace {
+
+void worker_loop(hb_server* S) {
+  (void)hipSetDevice(S->device);
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto set_error = [&](int rc) {
+    if (rc && S->error.empty()) S->error = hb_last_error();
+  };
+  for (;;) {
+    // retire finished launches
+    for (Slot& sl : S->slots)
+      if (sl.busy && hipEventQuery(sl.ev1) == hipSuccess) {
+        set_error(complete(S, sl));
+        S->cv_done.notify_all();
+      }
+    set_error(admit(S));
+    bool any_busy = S->slots[0].busy || S->slots[1].busy;
+    if (S->stop && S->queue.empty() && S->active.empty() && !any_busy) break;
+    if (!S->error.empty()) {
+      // a device error stops scheduling; waiters see it through hb_server_wait
+      S->cv_done.notify_all();
+      if (!any_busy) {
+        S->cv_work.wait(lk, [&] { return S->stop; });
+        break;
+      }
+    }
+    Slot* free_slot = !S->slots[0].busy ? &S->slots[0] : (!S->slots[1].busy ? &S->slots[1] : nullptr);
+    if (free_slot && S->error.empty()) {
+      set_error(launch(S, *free_slot));
+      if (free_slot->busy) continue;  // try to fill the other slot too
+    }
+    any_busy = S->slots[0].busy || S->slots[1].busy;
+    if (any_busy) {
+      // wait for the launch that started first
+      Slot* w = nullptr;
+      for (Slot& sl : S->slots)
+        if (sl.busy && (!w || sl.seq < w->seq)) w = &sl;
+      hipEvent_t ev = w->ev1;
+      lk.unlock();
+      (void)hipEventSynchronize(ev);
+      lk.lock();
+      continue;
+    }
+    S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+  }
+  S->cv_done.notify_all();
+}
+
+// Layers: inputs are layer 0, a gate's layer is 1 + the max layer of its inputs. Gates are
+// scheduled by (layer, kind) so each launch part has one gate kind.
+int build_request(const hb_circuit_t* c, uint32_t n1, Request* r) {
+  const uint32_t ni = c->n_inputs, ng = c->n_gates;
+  if (ni == 0 || ng == 0) return fail(HB_ERR_ARG, "circuit needs inputs and gates");
+  if (!c->in0 || !c->in1 || (c->n_outputs && !c->outputs))
+    return fail(HB_ERR_ARG, "circuit arrays missing");
+  std::vector<uint32_t> level(ni + ng, 0);
+  for (uint32_t g = 0; g < ng; g++) {
+    const uint32_t w = ni + g;
+    if (c->in0[g] >= w || c->in1[g] >= w)
+      return fail(HB_ERR_ARG, "gate %u reads wire %u/%u not defined before it", g, c->in0[g], c->in1[g]);
+    const uint8_t k = c->kind ? c->kind[g] : (uint8_t)GATE_NAND;
+    if (k > GATE_XNOR) return fail(HB_ERR_ARG, "gate %u: kind %u not a two-input gate", g, k);
+    level[w] = 1 + std::max(level[c->in0[g]], level[c->in1[g]]);
+  }
+  for (uint32_t k = 0; k < c->n_outputs; k++)
+    if (c->outputs[k] >= ni + ng) return fail(HB_ERR_ARG, "output %u out of range", k);
+  std::vector<uint32_t> order(ng);
+  for (uint32_t g = 0; g < ng; g++) order[g] = g;
+  auto kind_of = [&](uint32_t g) { return c->kind ? c->kind[g] : (uint8_t)GATE_NAND; };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    if (level[ni + a] != level[ni + b]) return level[ni + a] < level[ni + b];
+    return kind_of(a) < kind_of(b);
+  });
+  r->n1 = n1;
+  r->n_inputs = ni;
+  r->n_gates = ng;
+  r->idx.resize((size_t)3 * ng);
+  r->kind.resize(ng);
+  for (uint32_t s = 0; s < ng; s++) {
+    const uint32_t g = order[s];
+    r->idx[3 * (size_t)s] = c->in0[g];
+    r->idx[3 * (size_t)s + 1] = c->in1[g];
+    r->idx[3 * (size_t)s + 2] = ni + g;
+    r->kind[s] = kind_of(g);
+    const bool boundary = s + 1 == ng || level[ni + order[s + 1]] != level[ni + g] ||
+                          kind_of(order[s + 1]) != r->kind[s];
+    if (boundary) {
+      r->run_end.push_back(s + 1);
+      r->run_layer.push_back(level[ni + g]);
+    }
+  }
+  r->outputs.assign(c->outputs, c->outputs + c->n_outputs);
+  const size_t wires_bytes = (size_t)(ni + ng) * n1 * 8;
+  r->bytes = ((wires_bytes + 255) & ~(size_t)255) + (size_t)3 * ng * 4;
+  return HB_OK;
+}
+
+void release_slot(Slot& sl) {
+  if (sl.parts_dev) (void)hipFree(sl.parts_dev);
+  if (sl.parts_host) (void)hipHostFree(sl.parts_host);
+  if (sl.acc) (void)hipFree(sl.acc);
+  if (sl.ev0) (void)hipEventDestroy(sl.ev0);
+  if (sl.ev1) (void)hipEventDestroy(sl.ev1);
+  if (sl.s) (void)hipStreamDestroy(sl.s);
+  sl = Slot{};
+}
+
+}  // namespace
+
+extern "C" {
+
+int hb_server_create(int device, uint32_t max_gates, uint64_t hbm_budget, hb_server_t** out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  if (max_gates == 0 || max_gates > (1u << 20)) return fail(HB_ERR_ARG, "bad max_gates %u", max_gates);
+  int rc = ensure_engine(device);
+  if (rc) return rc;
+  std::unique_ptr<hb_server> S(new hb_server);
+  S->device = device;
+  S->max_gates = max_gates;
+  size_t fr = 0, tot = 0;
+  HIP_TRY(hipMemGetInfo(&fr, &tot));
+  const size_t slot_bytes = 2 * (size_t)max_gates * kAccPerGate;
+  if (hbm_budget == 0) hbm_budget = std::min<uint64_t>((fr > slot_bytes ? fr - slot_bytes : 0) / 2,
+                                                       32ull << 30);
+  if (hbm_budget + slot_bytes > fr)
+    return fail(HB_ERR_CAPACITY, "HBM budget %llu exceeds free device memory %zu",
+                (unsigned long long)hbm_budget, fr);
+  HIP_TRY(hipMalloc((void**)&S->arena_base, hbm_budget));
+  S->arena.init(hbm_budget);
+  for (Slot& sl : S->slots) {
+    HIP_TRY(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&sl.ev0));
+    HIP_TRY(hipEventCreate(&sl.ev1));
+    HIP_TRY(hipMalloc((void**)&sl.parts_dev, kMaxParts * sizeof(BootArgs)));
+    HIP_TRY(hipHostMalloc((void**)&sl.parts_host, kMaxParts * sizeof(BootArgs)));
+    HIP_TRY(hipMalloc(&sl.acc, (size_t)max_gates * kAccPerGate));
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+  S->st.budget = hbm_budget;
+  hb_server* raw = S.release();
+  raw->worker = std::thread(worker_loop, raw);
+  *out = raw;
+  return HB_OK;
+}
+
+int hb_server_destroy(hb_server_t* S) {
+  if (!S) return HB_OK;
+  {
+    std::lock_guard<std::mutex> g(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  for (Slot& sl : S->slots) release_slot(sl);
+  if (S->copy) (void)hipStreamDestroy(S->copy);
+  if (S->arena_base) (void)hipFree(S->arena_base);
+  delete S;
+  return HB_OK;
+}
+
+int hb_server_submit(hb_server_t* S, hb_ek_t* ek, const hb_circuit_t* c, const uint64_t* inputs,
+                     uint64_t* request_id) {
+  if (!S || !ek || !c || !inputs || !request_id) return fail(HB_ERR_ARG, "null argument");
+  if (ek->p.N != 1024) return fail(HB_ERR_ARG, "unsupported ring dimension");
+  std::unique_ptr<Request> r(new Request);
+  const uint32_t n1 = ek->p.n + 1;
+  int rc = build_request(c, n1, r.get());
+  if (rc) return rc;
+  if (r->n_gates > 0 && r->bytes > S->arena.size())
+    return fail(HB_ERR_CAPACITY, "circuit needs %zu bytes, HBM budget is %zu", r->bytes,
+                S->arena.size());
+  for (uint64_t k = 0; k < (uint64_t)c->n_inputs * n1; k++)
+    if (inputs[k] >= ek->p.q) return fail(HB_ERR_ARG, "input word %llu not canonical", (unsigned long long)k);
+  r->ek = ek;
+  r->inputs.assign(inputs, inputs + (size_t)c->n_inputs * n1);
+  r->t_submit = now_ms();
+  std::lock_guard<std::mutex> g(S->mu);
+  if (S->stop) return fail(HB_ERR_STATE, "server is shutting down");
+  r->id = S->next_id++;
+  *request_id = r->id;
+  S->queue.push_back(r.get());
+  S->reqs[r->id] = std::move(r);
+  S->st.submitted++;
+  S->st.queued = S->queue.size();
+  S->cv_work.notify_one();
+  return HB_OK;
+}
+
+int hb_server_poll(hb_server_t* S, uint64_t id, int* done) {
+  if (!S || !done) return fail(HB_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(HB_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+  *done = it->second->finished ? 1 : 0;
+  return S->error.empty() ? HB_OK : fail(HB_ERR_DEVICE, "%s", S->error.c_str());
+}
+
+int hb_server_wait(hb_server_t* S, uint64_t id, uint64_t* outputs, double* latency_ms) {
+  if (!S) return fail(HB_ERR_ARG, "null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(id);
+  if (it == S->reqs.end()) return fail(HB_ERR_ARG, "unknown request %llu", (unsigned long long)id);
+  Request* r = it->second.get();
+  S->cv_done.wait(lk, [&] { return r->finished || !S->error.empty(); });
+  if (!r->finished) return fail(HB_ERR_DEVICE, "server error: %s", S->error.c_str());
+  if (outputs) std::memcpy(outputs, r->out.data(), r->out.size() * 8);
+  if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  S->reqs.erase(it);
+  return HB_OK;
+}
+
+int hb_server_stats(hb_server_t* S, hb_server_stats_t* out) {
+  if (!S || !out) return fail(HB_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> g(S->mu);
+  *out = S->st;
+  out->bytes_in_use = S->arena.used();
+  out->queued = S->queue.size();
+  out->active = S->active.size();
+  return HB_OK;
+}
+
+}  // extern "C"

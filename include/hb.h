@@ -144,6 +144,50 @@ int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts);
 /* Device pointers; asynchronous on `stream` (NULL = the engine's stream). */
 int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream);
 
+/* ---- Serving loop (no reference counterpart: the reference evaluates one gate per call).
+ * A server admits boolean circuits from many clients (each with its own hb_ek_t) as they
+ * arrive, keeps their wires in HBM between layers, admits by an HBM budget (a device
+ * arena; requests that do not fit wait in FIFO order) and batches the ready gates of all
+ * admitted circuits into shared kernel launches (continuous batching, two launches in
+ * flight). ---- */
+typedef struct hb_server hb_server_t;
+
+/* Wires 0..n_inputs-1 are the inputs; gate g reads wires in0[g], in1[g] (each < n_inputs+g)
+ * and writes wire n_inputs+g. kind[g] is an hb_gate_kind of two inputs (NULL = all NAND). */
+typedef struct {
+  uint32_t n_inputs;
+  uint32_t n_gates;
+  uint32_t n_outputs;
+  uint32_t pad_;
+  const uint32_t* in0;
+  const uint32_t* in1;
+  const uint8_t* kind;
+  const uint32_t* outputs; /* wire indices returned by hb_server_wait */
+} hb_circuit_t;
+
+typedef struct {
+  uint64_t submitted, admitted, completed;
+  uint64_t gates_done, launches;
+  uint64_t queued, active;
+  uint64_t bytes_in_use, budget;
+  double busy_ms; /* summed event-timed kernel duration */
+} hb_server_stats_t;
+
+/* max_gates_per_launch: gate slots per launch; hbm_budget_bytes: arena size (0 = half of
+ * the free device memory, at most 32 GiB). */
+int hb_server_create(int device, uint32_t max_gates_per_launch, uint64_t hbm_budget_bytes,
+                     hb_server_t** out);
+int hb_server_destroy(hb_server_t* server);
+/* inputs: n_inputs LWE ciphertexts under ek's key. HB_ERR_CAPACITY if the circuit can
+ * never fit the budget. */
+int hb_server_submit(hb_server_t* server, hb_ek_t* ek, const hb_circuit_t* circuit,
+                     const uint64_t* inputs, uint64_t* request_id);
+int hb_server_poll(hb_server_t* server, uint64_t request_id, int* done);
+/* Blocks until done; outputs: n_outputs ciphertexts; latency = submit → done (ms). */
+int hb_server_wait(hb_server_t* server, uint64_t request_id, uint64_t* outputs,
+                   double* latency_ms);
+int hb_server_stats(hb_server_t* server, hb_server_stats_t* out);
+
 /* Standalone negacyclic NTT over `count` polynomials of N = 2^log_n words (device
  * pointers, canonical in and out, in place). Forward output in the key layout's slot
  * order. Replaces AbstractNTT::{transform_slice, inverse_transform_slice}

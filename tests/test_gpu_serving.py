@@ -1,0 +1,115 @@
+"""Serving loop: continuous batching of circuits from several clients.
+
+Each circuit's result must be identical (word for word) whether it runs alone or mixed
+with other clients' circuits, decrypt to the plaintext evaluation, and match the oracle
+gate by gate."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import stream
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def clients(hb, keyset):
+    out = []
+    for seed in (0, 1, 2):
+        P, sk, bsk, ksk = keyset(hb.SET_GOLDILOCKS, seed)
+        ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+        out.append((ev, hb.Encryptor(sk), hb.Decryptor(sk)))
+    return out
+
+
+def test_circuit_alone_equals_in_mix(hb, clients):
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    rng = stream(50, "serve")
+    jobs = []
+    for i in range(6):
+        ev, enc, dec = clients[i % 3]
+        circ = random_nand_circuit(rng, int(rng.integers(1, 7)), int(rng.integers(2, 13)))
+        bits = rng.integers(0, 2, size=circ.n_inputs)
+        jobs.append((ev, dec, circ, bits, enc.encrypt_many(bits)))
+    srv = Server(max_gates_per_launch=512)
+    alone = []
+    for ev, dec, circ, bits, cts in jobs:
+        out, _ = srv.wait(srv.submit(ev, circ, cts))
+        alone.append(out)
+    say("alone done")
+    ids = [srv.submit(ev, circ, cts) for ev, dec, circ, bits, cts in jobs]
+    mixed = [srv.wait(r)[0] for r in ids]
+    st = srv.stats()
+    srv.close()
+    say(f"stats {st}")
+    for (ev, dec, circ, bits, cts), a, m in zip(jobs, alone, mixed):
+        assert np.array_equal(a, m)
+        expect = circ.eval_plain(bits)
+        got = [dec.decrypt(hb.LWECiphertext(m[k])) for k in range(m.shape[0])]
+        assert list(expect) == got
+    assert st["completed"] == 12 and st["gates_done"] == 2 * sum(j[2].n_gates for j in jobs)
+
+
+def test_admission_by_hbm_budget(hb, clients):
+    """A budget that fits one circuit at a time: requests queue, then run in order, with the
+    same results as with a large budget."""
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    rng = stream(51, "budget")
+    ev, enc, dec = clients[0]
+    circs = [random_nand_circuit(rng, 3, 8) for _ in range(4)]
+    bits = [rng.integers(0, 2, size=8) for _ in circs]
+    cts = [enc.encrypt_many(b) for b in bits]
+    n1 = ev.params.n + 1
+    need = (8 + 24) * n1 * 8 + 256 + 3 * 24 * 4
+    small = Server(max_gates_per_launch=256, hbm_budget=need + 512)
+    big = Server(max_gates_per_launch=256, hbm_budget=64 << 20)
+    ids = [small.submit(ev, c, x) for c, x in zip(circs, cts)]
+    st_mid = small.stats()
+    outs_small = [small.wait(r)[0] for r in ids]
+    outs_big = [big.wait(big.submit(ev, c, x))[0] for c, x in zip(circs, cts)]
+    st = small.stats()
+    small.close()
+    big.close()
+    assert st_mid["queued"] + st_mid["active"] + st_mid["completed"] == 4
+    assert st["completed"] == 4 and st["admitted"] == 4
+    for a, b, c, bb in zip(outs_small, outs_big, circs, bits):
+        assert np.array_equal(a, b)
+        assert [dec.decrypt(hb.LWECiphertext(a[k])) for k in range(a.shape[0])] == list(c.eval_plain(bb))
+
+
+def test_oversized_circuit_is_rejected(hb, clients):
+    from hasteboots_amd._lib import HbError
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    ev, enc, dec = clients[0]
+    srv = Server(max_gates_per_launch=64, hbm_budget=1 << 20)
+    circ = random_nand_circuit(stream(52, "big"), 40, 40)
+    with pytest.raises(HbError) as e:
+        srv.submit(ev, circ, enc.encrypt_many(np.zeros(40, np.int64)))
+    assert e.value.code == -4
+    srv.close()
+
+
+def test_server_matches_oracle_gate_by_gate(hb, orc, keyset):
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    P, sk, bsk, ksk = keyset(hb.SET_BABYBEAR)
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+    rng = stream(53, "oracle")
+    circ = random_nand_circuit(rng, 3, 4)
+    bits = rng.integers(0, 2, size=4)
+    cts = hb.Encryptor(sk).encrypt_many(bits)
+    srv = Server(max_gates_per_launch=64)
+    out, _ = srv.wait(srv.submit(ev, circ, cts))
+    srv.close()
+    ok = orc.OracleKeys(hb.SET_BABYBEAR, bsk, ksk)
+    wires = list(cts)
+    for g in range(circ.n_gates):
+        r = ok.gate_batch(0, wires[circ.in0[g]][None, :], wires[circ.in1[g]][None, :])[0]
+        wires.append(r)
+    ref = np.stack([wires[o] for o in circ.outputs])
+    assert np.array_equal(out, ref)
