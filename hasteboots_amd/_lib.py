@@ -83,6 +83,12 @@ _PROTOS = {
     "hb_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
     "hb_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.POINTER(C.c_double)]),
     "hb_server_stats": (C.c_int, [C.c_void_p, C.POINTER(HbServerStats)]),
+    "hb_gate_batch_traced": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p,
+                                       u64p]),
+    "hb_bootstrap_traced": (C.c_int, [C.c_void_p, C.c_size_t, u64p, u64p, u64p, u64p]),
+    "hb_trace_sizes": (C.c_int, [C.c_int, u64p, u64p]),
+    "hb_trace_expand": (C.c_int, [C.c_void_p, C.c_int, u64p, u64p, u64p, u64p, u32p, C.c_uint32,
+                                  u64p, u64p]),
     "hb_mixed_batch": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t]),
     "hb_mixed_batch_dev": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t, C.c_void_p]),
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),

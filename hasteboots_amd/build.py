@@ -16,7 +16,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libhasteboots.so"
 
-SOURCES = ["hb_kernels.hip", "hb_api.cpp", "hb_keys.cpp", "hb_server.cpp"]
+SOURCES = ["hb_kernels.hip", "hb_api.cpp", "hb_keys.cpp", "hb_server.cpp", "hb_trace.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HB_OFFLOAD_ARCH", "gfx950")

@@ -523,6 +523,78 @@ int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts) {
   return record_ms(E.stream);
 }
 
+namespace {
+
+// Traced launch for dense inputs; gate GATE_BOOTSTRAP takes c0s as the ciphertexts and
+// `lut` (N canonical host words) as the shared test polynomial.
+int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,
+                 const uint64_t* c2s, const uint64_t* lut, uint64_t* outs, uint64_t* core_trace) {
+  Engine& E = engine();
+  std::lock_guard<std::mutex> g(ek->mu);
+  int rc = ensure_io(ek, count);
+  if (rc) return rc;
+  const ParamSet& p = ek->p;
+  if (lut) {
+    for (uint32_t k = 0; k < p.N; k++)
+      if (lut[k] >= p.q) return fail(HB_ERR_ARG, "LUT word %u not canonical", k);
+    HIP_TRY(hipMemcpyAsync(ek->lut_dev, lut, p.N * 8, hipMemcpyHostToDevice, E.stream));
+  }
+  const size_t n1 = p.n + 1, bytes = count * n1 * 8;
+  const size_t wb = p.is_goldilocks ? 8 : 4;
+  const size_t words = count * (size_t)(p.n + 1) * 2 * p.N;
+  void* tr = nullptr;
+  HIP_TRY(hipMalloc(&tr, words * wb));
+  uint64_t* d0 = ek->io_dev;
+  uint64_t* d1 = d0 + count * n1;
+  uint64_t* d2 = d1 + count * n1;
+  uint64_t* dout = d2 + count * n1;
+  auto cleanup = [&](int code) {
+    (void)hipFree(tr);
+    return code;
+  };
+  if (hipMemcpyAsync(d0, c0s, bytes, hipMemcpyHostToDevice, E.stream) != hipSuccess ||
+      (c1s && hipMemcpyAsync(d1, c1s, bytes, hipMemcpyHostToDevice, E.stream) != hipSuccess) ||
+      (c2s && hipMemcpyAsync(d2, c2s, bytes, hipMemcpyHostToDevice, E.stream) != hipSuccess))
+    return cleanup(fail(HB_ERR_DEVICE, "input copy failed"));
+  BootArgs a = make_args(ek, gate, count, d0, c1s ? d1 : nullptr, c2s ? d2 : nullptr,
+                         lut ? (const uint64_t*)ek->lut_dev : nullptr, dout, ek->acc_dev);
+  a.trace = tr;
+  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, E.stream) != hipSuccess ||
+      hipEventRecord(E.ev1, E.stream) != hipSuccess)
+    return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
+  std::vector<uint8_t> host(words * wb);
+  if (hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
+      hipMemcpyAsync(host.data(), tr, words * wb, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
+      hipStreamSynchronize(E.stream) != hipSuccess)
+    return cleanup(fail(HB_ERR_DEVICE, "trace copy failed"));
+  if (p.is_goldilocks) {
+    std::memcpy(core_trace, host.data(), words * 8);
+  } else {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(host.data());
+    for (size_t k = 0; k < words; k++) core_trace[k] = w[k];
+  }
+  return cleanup(record_ms(E.stream));
+}
+
+}  // namespace
+
+int hb_gate_batch_traced(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s,
+                         const uint64_t* c1s, const uint64_t* c2s, uint64_t* outs,
+                         uint64_t* core_trace) {
+  if (!ek || !c0s || !c1s || !outs || !core_trace) return fail(HB_ERR_ARG, "null argument");
+  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (gate == GATE_MAJORITY && !c2s) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (count == 0) return HB_OK;
+  return traced_batch(ek, gate, count, c0s, c1s, c2s, nullptr, outs, core_trace);
+}
+
+int hb_bootstrap_traced(hb_ek_t* ek, size_t count, const uint64_t* cs, const uint64_t* lut,
+                        uint64_t* outs, uint64_t* core_trace) {
+  if (!ek || !cs || !lut || !outs || !core_trace) return fail(HB_ERR_ARG, "null argument");
+  if (count == 0) return HB_OK;
+  return traced_batch(ek, GATE_BOOTSTRAP, count, cs, nullptr, nullptr, lut, outs, core_trace);
+}
+
 int hb_nand(hb_ek_t* ek, const uint64_t* c0, const uint64_t* c1, uint64_t* out) {
   return hb_gate_batch(ek, GATE_NAND, 1, c0, c1, nullptr, out);
 }

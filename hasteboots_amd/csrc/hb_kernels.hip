@@ -162,7 +162,12 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   const W* __restrict__ itw = static_cast<const W*>(A.tw_inv);
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
   const W* __restrict__ ksk = static_cast<const W*>(A.ksk);
-  W* accA = static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  // Untraced: one accumulator per gate, updated in place. Traced (the vfhe path): an
+  // (n+1)-slot history per gate; round i reads slot i and writes slot i+1, so the
+  // history is the core trace (the acc entering every round, then the final acc).
+  const bool traced = A.trace != nullptr;
+  W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N
+                   : static_cast<W*>(A.acc) + (size_t)g * 2 * N;
   W* accB = accA + N;
 
   // 1. gate combination + modulus switch of all n+1 coefficients.
@@ -211,9 +216,23 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 
   // 3. blind rotation: for a_i != 0, ACC += ExtProd((X^a_i - 1) ACC, RGSW(s_i)).
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
-  for (u32 i = 0; i < n; i++) {
+  const size_t step = traced ? 2 * N : 0;
+  for (u32 i = 0; i < n; i++, accA += step, accB += step) {
     const int r = ms[i];
-    if (r == 0) continue;  // binary.rs:70
+    W* outA = accA + step;
+    W* outB = accB + step;
+    if (r == 0) {  // binary.rs:70 skips the round; the vfhe path adds a zero product
+      if (traced) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int k = lane + 64 * e;
+          outA[k] = accA[k];
+          outB[k] = accB[k];
+        }
+        __syncthreads();
+      }
+      continue;
+    }
     const W* key = bsk + (size_t)i * rgsw_words;
     W pa[E], pb[E];
 #pragma unroll
@@ -238,7 +257,9 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
           src = neg ? k - rr : k - rr + N;
         }
         W v = acc[src];
-        if (neg) v = F::neg_raw(v);
+        // untraced: the reference's raw-word negation; traced: the NTT product
+        // ACC * X^r (binary.rs:158-181) is exact, i.e. negation of the canonical value
+        if (neg) v = traced ? F::neg_raw(F::canon(v)) : F::neg_raw(v);
         dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
       }
       // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
@@ -251,13 +272,13 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
-      accA[k] = F::add(accA[k], pa[e]);
+      outA[k] = F::add(accA[k], pa[e]);
     }
     ntt_inv_wave<F, E>(pb, lds, itw, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
-      accB[k] = F::add(accB[k], pb[e]);
+      outB[k] = F::add(accB[k], pb[e]);
     }
     __syncthreads();
   }

@@ -39,6 +39,7 @@ struct BootArgs {
   int is_goldilocks;
   uint32_t first_block;  // mixed launches: first workgroup index of this part
   const uint32_t* idx;   // optional: per gate (in0, in1, out) wire rows of c0 / out
+  void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
 };
 
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s);

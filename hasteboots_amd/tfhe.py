@@ -278,6 +278,50 @@ class Evaluator:
     def nand_batch(self, c0s: np.ndarray, c1s: np.ndarray) -> np.ndarray:
         return self.gate_batch("nand", c0s, c1s)
 
+    # ---- traced gates (vfhe::Evaluator, vfhe/src/bfhe/evaluate.rs:47-139) ----
+    def gate_batch_traced(self, gate: str, c0s, c1s, c2s=None):
+        """Gates with the traced path's semantics; returns (outputs, core traces) where the
+        core trace of gate g is the accumulator entering each round plus the final one,
+        shape (count, n+1, 2, N)."""
+        c0s = np.ascontiguousarray(c0s, np.uint64)
+        c1s = np.ascontiguousarray(c1s, np.uint64)
+        c2s = None if c2s is None else np.ascontiguousarray(c2s, np.uint64)
+        P = self.params
+        out = np.zeros_like(c0s)
+        core = np.zeros((c0s.shape[0], P.n + 1, 2, P.N), dtype=np.uint64)
+        check(lib().hb_gate_batch_traced(self._h, GATES[gate], c0s.shape[0], ptr(c0s), ptr(c1s),
+                                         ptr(c2s), ptr(out), ptr(core)))
+        return out, core
+
+    def bootstrap_traced(self, cs, lut):
+        """Traced bootstrap of ciphertexts cs (count, n+1) with one LUT; (outputs, cores)."""
+        cs = np.ascontiguousarray(cs, np.uint64)
+        lut = np.ascontiguousarray(lut, np.uint64)
+        P = self.params
+        out = np.zeros_like(cs)
+        core = np.zeros((cs.shape[0], P.n + 1, 2, P.N), dtype=np.uint64)
+        check(lib().hb_bootstrap_traced(self._h, cs.shape[0], ptr(cs), ptr(lut), ptr(out),
+                                        ptr(core)))
+        return out, core
+
+    def expand_trace(self, gate: str, c0, c1, core, rounds, c2=None, with_tail: bool = True):
+        """PBSTrace components of one gate rebuilt from its core trace: one record per listed
+        round and the tail record (modulus switch, key switch, sample extraction)."""
+        rw, tw = C.c_uint64(), C.c_uint64()
+        check(lib().hb_trace_sizes(self.params.set_id, C.byref(rw), C.byref(tw)))
+        rounds = np.ascontiguousarray(rounds, np.uint32)
+        rec = np.zeros((rounds.size, rw.value), dtype=np.uint64)
+        tail = np.zeros(tw.value, dtype=np.uint64) if with_tail else None
+        core = np.ascontiguousarray(core, np.uint64)
+        c0 = np.ascontiguousarray(c0, np.uint64)
+        c1 = None if c1 is None else np.ascontiguousarray(c1, np.uint64)
+        c2 = None if c2 is None else np.ascontiguousarray(c2, np.uint64)
+        gate_id = 7 if gate == "bootstrap" else GATES[gate]
+        check(lib().hb_trace_expand(self._h, gate_id, ptr(c0), ptr(c1), ptr(c2), ptr(core),
+                                    rounds.ctypes.data_as(C.POINTER(C.c_uint32)), rounds.size,
+                                    ptr(rec), ptr(tail)))
+        return rec, tail
+
 
 def mixed_batch(parts) -> list[np.ndarray]:
     """Bootstrap gates of several evaluation keys (any parameter sets) in ONE launch.

@@ -144,6 +144,30 @@ int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts);
 /* Device pointers; asynchronous on `stream` (NULL = the engine's stream). */
 int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream);
 
+/* ---- Traced bootstrapping: vfhe::Evaluator (vfhe/src/bfhe/evaluate.rs:47-139, 196-208),
+ * whose bootstrap returns a PBSTrace for the prover (trace/src/pbs_trace.rs:8-13). ----
+ * Same gates as hb_gate_batch with the traced path's semantics (monomial products through
+ * the NTT, every round executed), plus a compact core trace per gate: the accumulator
+ * RLWE entering each of the n rounds and the final one, (n+1)*2N words per gate (raw
+ * Goldilocks words / BabyBear values), a then b per slot. */
+int hb_gate_batch_traced(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s,
+                         const uint64_t* c1s, const uint64_t* c2s, uint64_t* outs,
+                         uint64_t* core_trace);
+/* Traced EvaluationKey::bootstrap with one caller LUT (N canonical words) shared by the
+ * `count` ciphertexts in cs. */
+int hb_bootstrap_traced(hb_ek_t* ek, size_t count, const uint64_t* cs, const uint64_t* lut,
+                        uint64_t* outs, uint64_t* core_trace);
+/* Words of one expanded round record and of the tail record (layouts in hb_trace.cpp).
+ * hb_trace_expand takes gate = 7 with c1 = c2 = NULL for hb_bootstrap_traced gates. */
+int hb_trace_sizes(int set, uint64_t* round_words, uint64_t* tail_words);
+/* Rebuilds the PBSTrace components of one gate from its core trace: the listed rounds
+ * (increasing, < n) into rec, and the modulus-switch / key-switch / sample-extraction
+ * values into tail (NULL skips). Checks the core trace against the keys on the way
+ * (HB_ERR_STATE if a round's product does not match). */
+int hb_trace_expand(const hb_ek_t* ek, int gate, const uint64_t* c0, const uint64_t* c1,
+                    const uint64_t* c2, const uint64_t* core_trace, const uint32_t* rounds,
+                    uint32_t n_rounds, uint64_t* rec, uint64_t* tail);
+
 /* ---- Serving loop (no reference counterpart: the reference evaluates one gate per call).
  * A server admits boolean circuits from many clients (each with its own hb_ek_t) as they
  * arrive, keeps their wires in HBM between layers, admits by an HBM budget (a device

@@ -60,6 +60,9 @@ def lib() -> C.CDLL:
                                      C.c_int]
         l.orc_build_lut.argtypes = [C.POINTER(Params), C.c_int, u64p]
         l.orc_decrypt_with_noise.argtypes = [C.POINTER(Params), u64p, u64p, u64p, u64p]
+        l.orc_trace_sizes.argtypes = [C.POINTER(Params), u64p, u64p]
+        l.orc_bootstrap_traced.argtypes = [C.c_void_p, u64p, u64p, u64p, u32p, C.c_uint32, u64p,
+                                           u64p]
         l.orc_raw_neg_events.argtypes = [C.c_int]
         l.orc_raw_neg_events.restype = C.c_uint64
         _lib = l
@@ -188,6 +191,25 @@ class OracleKeys:
         out = np.zeros(self.p.n + 1, dtype=np.uint64)
         assert lib().orc_bootstrap(self.h, _p(c), _p(lut), _p(out)) == 0
         return out
+
+    def trace_sizes(self) -> tuple[int, int]:
+        rw, tw = C.c_uint64(), C.c_uint64()
+        lib().orc_trace_sizes(C.byref(self.p), C.cast(C.byref(rw), u64p), C.cast(C.byref(tw), u64p))
+        return int(rw.value), int(tw.value)
+
+    def bootstrap_traced(self, c, lut, rounds):
+        """The traced (vfhe) bootstrap: (output, round records, tail record)."""
+        c = np.ascontiguousarray(c, np.uint64)
+        lut = np.ascontiguousarray(lut, np.uint64)
+        rounds = np.ascontiguousarray(rounds, np.uint32)
+        rw, tw = self.trace_sizes()
+        rec = np.zeros((rounds.size, rw), dtype=np.uint64)
+        tail = np.zeros(tw, dtype=np.uint64)
+        out = np.zeros(self.p.n + 1, dtype=np.uint64)
+        assert lib().orc_bootstrap_traced(self.h, _p(c), _p(lut), _p(out),
+                                          rounds.ctypes.data_as(u32p), rounds.size, _p(rec),
+                                          _p(tail)) == 0
+        return out, rec, tail
 
     def blind_rotate(self, ms, lut) -> np.ndarray:
         ms = np.ascontiguousarray(ms, np.uint32)

@@ -349,6 +349,201 @@ static void bootstrap(const Params& p, const typename F::word* bsk, const typena
   key_switch<F>(p, ksk, s.acc_a.data(), b0, out, s, T);
 }
 
+// ---------------------------------------------------------------------------
+// Traced twin: vfhe::EvaluationKey::bootstrap (vfhe/src/bfhe/evaluate.rs:47-139) with
+// blind_rotate_w_trace (fhe_core/src/blind_rotation/rlwe/binary.rs:106-204),
+// mul_assign_ntt_rgsw_w_trace (lattice/src/rlwe.rs:604-632) and
+// key_switch_for_rlwe_w_trace (fhe_core/src/key_switch/rlwe.rs:143-198, 290-350).
+// Differences from the untraced path: the LUT rotation and (X^a_i)·ACC go through NTT
+// products (exact, no raw-word negation), and every round runs (no a_i == 0 skip).
+//
+// Trace records hold canonical values. Round record (R = (22 + 12*br_len)*N + 2 words):
+//   input_acc [a, b, NTT a, NTT b] 4N | output_acc 4N | monomial [poly, NTT] 2N |
+//   monomial representation [degree, coefficient] 2 | monomial_times_acc 4N |
+//   external_product_input 4N | 2*br_len hadamards [bit poly, bit NTT, rlwe NTT a, b,
+//   rlwe poly a, b] 6N each | sum_prod [NTT a, NTT b, poly a, poly b] 4N.
+// Tail record: ms input (n+1) | ms output (n+1) | ks decomposed poly N |
+//   ks_len hadamards 6N each | ks sum_prod 4N | sample extraction input N, output N.
+// (trace/src/acc_trace.rs:104-138, basic_ops/hadamard_trace.rs:329-394,
+//  basic_ops/rlwe_trace.rs:146-218, modulus_switching_trace.rs:100-108, pbs_trace.rs:8-13.)
+// ---------------------------------------------------------------------------
+template <class F>
+static void put_poly(u64* dst, const typename F::word* src, u32 N) {
+  for (u32 k = 0; k < N; k++) dst[k] = F::value(src[k]);
+}
+template <class F>
+static void put_poly_ntt(u64* dst, const typename F::word* src, u32 N, const NttTable<F>& T) {
+  std::vector<typename F::word> t(src, src + N);
+  ntt_fwd<F>(t.data(), T);
+  put_poly<F>(dst, t.data(), N);
+}
+template <class F>
+static void put_rlwe(u64* dst, const typename F::word* a, const typename F::word* b, u32 N,
+                     const NttTable<F>& T) {
+  put_poly<F>(dst, a, N);
+  put_poly<F>(dst + N, b, N);
+  put_poly_ntt<F>(dst + 2 * N, a, N, T);
+  put_poly_ntt<F>(dst + 3 * N, b, N, T);
+}
+
+static size_t trace_round_words(const Params& p) {
+  return (size_t)(22 + 12 * p.br_len) * p.N + 2;
+}
+static size_t trace_tail_words(const Params& p) {
+  return (size_t)2 * (p.n + 1) + (size_t)p.N + (size_t)6 * p.ks_len * p.N + 4 * p.N + 2 * p.N;
+}
+
+// Traced gadget product: med += Σ_j NTT(digit_j(src)) ⊙ key_j, recording each hadamard
+// (mul_polynomial_inplace_fast_w_traces, lattice/src/gadget/rlwe.rs:426-457).
+template <class F>
+static void gadget_mac_traced(const typename F::word* src, const typename F::word* key, u32 len,
+                              u32 bits, Scratch<F>& s, const NttTable<F>& T, u64* had) {
+  using W = typename F::word;
+  const u32 N = T.N;
+  const u64 mask = (1ull << bits) - 1;
+  std::copy(src, src + N, s.poly.begin());
+  for (u32 j = 0; j < len; j++) {
+    for (u32 k = 0; k < N; k++) decompose_lsb_bits_at<F>(s.poly[k], s.digit[k], mask, bits);
+    u64* h = had ? had + (size_t)6 * j * N : nullptr;
+    if (h) put_poly<F>(h, s.digit.data(), N);
+    ntt_fwd<F>(s.digit.data(), T);
+    if (h) put_poly<F>(h + N, s.digit.data(), N);
+    const W* ka = key + (size_t)(2 * j) * N;
+    const W* kb = ka + N;
+    for (u32 k = 0; k < N; k++) {
+      s.med_a[k] = F::add(s.med_a[k], F::mul(ka[k], s.digit[k]));
+      s.med_b[k] = F::add(s.med_b[k], F::mul(kb[k], s.digit[k]));
+    }
+    if (h) {  // HadamardTrace::append_rlwe_ntt: the key row and its inverse transform
+      put_poly<F>(h + 2 * N, ka, N);
+      put_poly<F>(h + 3 * N, kb, N);
+      std::vector<W> ia(ka, ka + N), ib(kb, kb + N);
+      ntt_inv<F>(ia.data(), T);
+      ntt_inv<F>(ib.data(), T);
+      put_poly<F>(h + 4 * N, ia.data(), N);
+      put_poly<F>(h + 5 * N, ib.data(), N);
+    }
+  }
+}
+
+// rounds: sorted indices of the rounds to record into rec (one round record each).
+template <class F>
+static void bootstrap_traced(const Params& p, const typename F::word* bsk,
+                             const typename F::word* ksk, const typename F::word* c,
+                             const typename F::word* lut, typename F::word* out,
+                             const u32* rounds, u32 n_rounds, u64* rec, u64* tail, Scratch<F>& s,
+                             const NttTable<F>& T) {
+  using W = typename F::word;
+  const u32 N = p.N, n = p.n;
+  const size_t R = trace_round_words(p);
+  // lwe_modulus_switch_w_trace (fhe_core/src/modulus_switch.rs:41-86)
+  std::vector<u32> ms(n + 1);
+  for (u32 k = 0; k <= n; k++) {
+    ms[k] = mod_switch_one(F::value(c[k]), p.q, 2ull * N);
+    if (tail) tail[k] = F::value(c[k]), tail[n + 1 + k] = ms[k];
+  }
+  // LUT * X^-b through NTT products (binary.rs:126-143)
+  std::vector<W> l(lut, lut + N), m(N);
+  if (ms[n] != 0) {
+    u32 neg_b = 2 * N - ms[n];
+    std::fill(m.begin(), m.end(), F::from_value(0));
+    if (neg_b < N) m[neg_b] = F::from_value(1);
+    else m[neg_b - N] = F::from_value(p.q - 1);
+    ntt_fwd<F>(m.data(), T);
+    ntt_fwd<F>(l.data(), T);
+    for (u32 k = 0; k < N; k++) l[k] = F::mul(l[k], m[k]);
+    ntt_inv<F>(l.data(), T);
+  }
+  std::fill(s.acc_a.begin(), s.acc_a.end(), F::from_value(0));
+  std::copy(l.begin(), l.end(), s.acc_b.begin());
+  const size_t rgsw_words = (size_t)2 * p.br_len * 2 * N;
+  std::vector<W> na(N), nb(N), pa(N), pb(N);
+  u32 ri = 0;
+  for (u32 i = 0; i < n; i++) {
+    u64* r = (ri < n_rounds && rounds[ri] == i) ? rec + (size_t)(ri++) * R : nullptr;
+    if (r) put_rlwe<F>(r, s.acc_a.data(), s.acc_b.data(), N, T);
+    // product = ACC * X^a_i via NTT (binary.rs:158-178)
+    std::copy(s.acc_a.begin(), s.acc_a.end(), na.begin());
+    std::copy(s.acc_b.begin(), s.acc_b.end(), nb.begin());
+    ntt_fwd<F>(na.data(), T);
+    ntt_fwd<F>(nb.data(), T);
+    const u32 a_i = ms[i];
+    std::fill(m.begin(), m.end(), F::from_value(0));
+    u32 deg;
+    u64 coef;
+    if (a_i < N) deg = a_i, coef = 1;
+    else deg = a_i - N, coef = p.q - 1;
+    m[deg] = F::from_value(coef);
+    if (r) {
+      put_poly<F>(r + 8 * N, m.data(), N);
+      r[10 * N] = deg;
+      r[10 * N + 1] = coef;
+    }
+    ntt_fwd<F>(m.data(), T);
+    if (r) put_poly<F>(r + 9 * N, m.data(), N);
+    for (u32 k = 0; k < N; k++) pa[k] = F::mul(na[k], m[k]), pb[k] = F::mul(nb[k], m[k]);
+    ntt_inv<F>(pa.data(), T);
+    ntt_inv<F>(pb.data(), T);
+    if (r) put_rlwe<F>(r + 10 * N + 2, pa.data(), pb.data(), N, T);
+    // external_product.sub_assign_element_wise(&acc) (binary.rs:180)
+    for (u32 k = 0; k < N; k++) {
+      s.ext_a[k] = F::sub(pa[k], s.acc_a[k]);
+      s.ext_b[k] = F::sub(pb[k], s.acc_b[k]);
+    }
+    if (r) put_rlwe<F>(r + 14 * N + 2, s.ext_a.data(), s.ext_b.data(), N, T);
+    std::fill(s.med_a.begin(), s.med_a.end(), F::from_value(0));
+    std::fill(s.med_b.begin(), s.med_b.end(), F::from_value(0));
+    const W* key = bsk + i * rgsw_words;
+    u64* had = r ? r + 18 * N + 2 : nullptr;
+    gadget_mac_traced<F>(s.ext_a.data(), key, p.br_len, p.br_bits, s, T, had);
+    gadget_mac_traced<F>(s.ext_b.data(), key + (size_t)2 * p.br_len * N, p.br_len, p.br_bits, s,
+                         T, had ? had + (size_t)6 * p.br_len * N : nullptr);
+    u64* sp = r ? r + (size_t)(18 + 12 * p.br_len) * N + 2 : nullptr;
+    if (sp) put_poly<F>(sp, s.med_a.data(), N), put_poly<F>(sp + N, s.med_b.data(), N);
+    std::copy(s.med_a.begin(), s.med_a.end(), s.ext_a.begin());
+    std::copy(s.med_b.begin(), s.med_b.end(), s.ext_b.begin());
+    ntt_inv<F>(s.ext_a.data(), T);
+    ntt_inv<F>(s.ext_b.data(), T);
+    if (sp) put_poly<F>(sp + 2 * N, s.ext_a.data(), N), put_poly<F>(sp + 3 * N, s.ext_b.data(), N);
+    for (u32 k = 0; k < N; k++) {
+      s.acc_a[k] = F::add(s.acc_a[k], s.ext_a[k]);
+      s.acc_b[k] = F::add(s.acc_b[k], s.ext_b[k]);
+    }
+    if (r) put_rlwe<F>(r + 4 * N, s.acc_a.data(), s.acc_b.data(), N, T);
+  }
+  // acc.b[0] += Q/8 (vfhe/src/bfhe/evaluate.rs:91) and the traced key switch.
+  W b0 = F::add(s.acc_b[0], F::from_value(p.q >> 3));
+  std::fill(s.med_a.begin(), s.med_a.end(), F::from_value(0));
+  std::fill(s.med_b.begin(), s.med_b.end(), F::from_value(0));
+  std::vector<W> negated(N);
+  for (u32 k = 0; k < N; k++) negated[k] = F::neg(s.acc_a[k]);
+  u64* t = tail ? tail + 2 * (n + 1) : nullptr;
+  if (t) put_poly<F>(t, negated.data(), N);  // decomposed_polys[0].poly (its ntt stays zero)
+  gadget_mac_traced<F>(negated.data(), ksk, p.ks_len, p.ks_bits, s, T, t ? t + N : nullptr);
+  // sum_prod = init - init_temp: the MAC alone, since init was (0, b0) (rlwe.rs:335-341)
+  u64* sp = t ? t + N + (size_t)6 * p.ks_len * N : nullptr;
+  std::vector<W> sa(s.med_a), sb(s.med_b);
+  if (sp) {
+    put_poly<F>(sp, sa.data(), N);
+    put_poly<F>(sp + N, sb.data(), N);
+    ntt_inv<F>(sa.data(), T);
+    ntt_inv<F>(sb.data(), T);
+    put_poly<F>(sp + 2 * N, sa.data(), N);
+    put_poly<F>(sp + 3 * N, sb.data(), N);
+  }
+  for (u32 k = 0; k < N; k++) s.med_b[k] = F::add(s.med_b[k], b0);
+  ntt_inv<F>(s.med_a.data(), T);
+  ntt_inv<F>(s.med_b.data(), T);
+  // extract_partial_lwe_locally_w_trace (lattice/src/rlwe.rs:406-434)
+  std::vector<W> a(s.med_a.begin(), s.med_a.end());
+  if (sp) put_poly<F>(sp + 4 * N, a.data(), N);
+  std::reverse(a.begin() + 1, a.end());
+  for (u32 k = 1; k < N; k++) a[k] = F::neg(a[k]);
+  if (sp) put_poly<F>(sp + 5 * N, a.data(), N);
+  for (u32 k = 0; k < n; k++) out[k] = a[k];
+  out[n] = s.med_b[0];
+}
+
 // LUT builders (tfhe/src/bfhe/evaluate.rs:267-465), lut_step = 1.
 enum Lut { LUT_NAND = 0, LUT_AND_MAJ = 1, LUT_OR = 2, LUT_NOR = 3, LUT_XOR = 4, LUT_XNOR = 5 };
 template <class F>
@@ -667,6 +862,41 @@ int orc_bootstrap(void* ek, const u64* c, const u64* lut, u64* out) {
     for (u32 k = 0; k < N; k++) l[k] = Bb::from_value(lut[k]);
     bootstrap<Bb>(p, e->bsk.data(), e->ksk.data(), cc.data(), l.data(), o.data(), s,
                   table<Bb>(log2u(N)));
+    for (u32 k = 0; k < n1; k++) out[k] = Bb::value(o[k]);
+  }
+  return 0;
+}
+
+// Sizes (words) of one round record and of the tail record of the traced bootstrap.
+int orc_trace_sizes(const Params* p, u64* round_words, u64* tail_words) {
+  *round_words = trace_round_words(*p);
+  *tail_words = trace_tail_words(*p);
+  return 0;
+}
+
+// Traced bootstrap (the vfhe path) with an arbitrary LUT; records the listed rounds
+// (sorted, < n) into rec and the modulus-switch / key-switch / extraction values into
+// tail (either may be NULL). out: n+1 canonical words.
+int orc_bootstrap_traced(void* ek, const u64* c, const u64* lut, u64* out, const u32* rounds,
+                         u32 n_rounds, u64* rec, u64* tail) {
+  EkBase* b = static_cast<EkBase*>(ek);
+  const Params& p = b->p;
+  const u32 N = p.N, n1 = p.n + 1;
+  if (p.is_goldilocks) {
+    auto* e = static_cast<Ek<Gl>*>(b);
+    Scratch<Gl> s(N);
+    std::vector<u64> o(n1);
+    bootstrap_traced<Gl>(p, e->bsk.data(), e->ksk.data(), c, lut, o.data(), rounds, n_rounds, rec,
+                         tail, s, table<Gl>(log2u(N)));
+    for (u32 k = 0; k < n1; k++) out[k] = Gl::value(o[k]);
+  } else {
+    auto* e = static_cast<Ek<Bb>*>(b);
+    Scratch<Bb> s(N);
+    std::vector<u32> cc(n1), l(N), o(n1);
+    for (u32 k = 0; k < n1; k++) cc[k] = Bb::from_value(c[k]);
+    for (u32 k = 0; k < N; k++) l[k] = Bb::from_value(lut[k]);
+    bootstrap_traced<Bb>(p, e->bsk.data(), e->ksk.data(), cc.data(), l.data(), o.data(), rounds,
+                         n_rounds, rec, tail, s, table<Bb>(log2u(N)));
     for (u32 k = 0; k < n1; k++) out[k] = Bb::value(o[k]);
   }
   return 0;
