@@ -80,6 +80,10 @@ _PROTOS = {
     "hb_server_destroy": (C.c_int, [C.c_void_p]),
     "hb_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(HbCircuit), u64p,
                                    C.POINTER(C.c_uint64)]),
+    "hb_server_submit_traced": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(HbCircuit), u64p,
+                                          C.c_int, C.POINTER(C.c_uint64)]),
+    "hb_server_wait_traced": (C.c_int, [C.c_void_p, C.c_uint64, u64p, u64p,
+                                        C.POINTER(C.c_double)]),
     "hb_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)]),
     "hb_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.POINTER(C.c_double)]),
     "hb_server_stats": (C.c_int, [C.c_void_p, C.POINTER(HbServerStats)]),

@@ -92,6 +92,11 @@ struct Request {
   uint32_t next = 0;       // first gate not yet launched
   uint32_t completed = 0;  // gates finished
   bool in_flight = false, finished = false;
+  // traced requests (vfhe semantics): per gate in schedule order, the (n+1)*2N-word core
+  // trace, kept in the arena at trace_off and copied out at completion
+  bool traced = false;
+  size_t trace_off = 0, trace_gate_bytes = 0;
+  std::vector<uint8_t> trace_host;
   std::vector<uint64_t> out;
   double t_submit = 0, t_admit = 0, t_done = 0;
 };
@@ -166,6 +171,11 @@ int finish(hb_server* S, Request* r) {
   for (size_t k = 0; k < r->outputs.size(); k++)
     HIP_TRY(hipMemcpyAsync(r->out.data() + k * r->n1, base + (size_t)r->outputs[k] * r->n1 * 8,
                            (size_t)r->n1 * 8, hipMemcpyDeviceToHost, S->copy));
+  if (r->traced) {
+    r->trace_host.resize(r->trace_gate_bytes * r->n_gates);
+    HIP_TRY(hipMemcpyAsync(r->trace_host.data(), base + r->trace_off, r->trace_host.size(),
+                           hipMemcpyDeviceToHost, S->copy));
+  }
   HIP_TRY(hipStreamSynchronize(S->copy));
   S->arena.release(r->off, r->bytes);
   r->finished = true;
@@ -213,6 +223,7 @@ int launch(hb_server* S, Slot& sl) {
                              (char*)sl.acc + (size_t)used * kAccPerGate);
       a.idx = idx + 3 * (size_t)pos;
       a.first_block = used;
+      if (r->traced) a.trace = S->arena_base + r->off + r->trace_off + (size_t)pos * r->trace_gate_bytes;
       sl.parts_host[nparts++] = a;
       used += cnt;
       pos += cnt;
@@ -400,14 +411,20 @@ int hb_server_destroy(hb_server_t* S) {
   return HB_OK;
 }
 
-int hb_server_submit(hb_server_t* S, hb_ek_t* ek, const hb_circuit_t* c, const uint64_t* inputs,
-                     uint64_t* request_id) {
+int hb_server_submit_traced(hb_server_t* S, hb_ek_t* ek, const hb_circuit_t* c,
+                            const uint64_t* inputs, int traced, uint64_t* request_id) {
   if (!S || !ek || !c || !inputs || !request_id) return fail(HB_ERR_ARG, "null argument");
   if (ek->p.N != 1024) return fail(HB_ERR_ARG, "unsupported ring dimension");
   std::unique_ptr<Request> r(new Request);
   const uint32_t n1 = ek->p.n + 1;
   int rc = build_request(c, n1, r.get());
   if (rc) return rc;
+  if (traced) {
+    r->traced = true;
+    r->trace_gate_bytes = (size_t)n1 * 2 * ek->p.N * (ek->p.is_goldilocks ? 8 : 4);
+    r->trace_off = (r->bytes + 255) & ~(size_t)255;
+    r->bytes = r->trace_off + r->trace_gate_bytes * r->n_gates;
+  }
   if (r->n_gates > 0 && r->bytes > S->arena.size())
     return fail(HB_ERR_CAPACITY, "circuit needs %zu bytes, HBM budget is %zu", r->bytes,
                 S->arena.size());
@@ -428,6 +445,11 @@ int hb_server_submit(hb_server_t* S, hb_ek_t* ek, const hb_circuit_t* c, const u
   return HB_OK;
 }
 
+int hb_server_submit(hb_server_t* S, hb_ek_t* ek, const hb_circuit_t* c, const uint64_t* inputs,
+                     uint64_t* request_id) {
+  return hb_server_submit_traced(S, ek, c, inputs, 0, request_id);
+}
+
 int hb_server_poll(hb_server_t* S, uint64_t id, int* done) {
   if (!S || !done) return fail(HB_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> g(S->mu);
@@ -437,7 +459,8 @@ int hb_server_poll(hb_server_t* S, uint64_t id, int* done) {
   return S->error.empty() ? HB_OK : fail(HB_ERR_DEVICE, "%s", S->error.c_str());
 }
 
-int hb_server_wait(hb_server_t* S, uint64_t id, uint64_t* outputs, double* latency_ms) {
+int hb_server_wait_traced(hb_server_t* S, uint64_t id, uint64_t* outputs,
+                          uint64_t* core_traces, double* latency_ms) {
   if (!S) return fail(HB_ERR_ARG, "null server");
   std::unique_lock<std::mutex> lk(S->mu);
   auto it = S->reqs.find(id);
@@ -447,8 +470,27 @@ int hb_server_wait(hb_server_t* S, uint64_t id, uint64_t* outputs, double* laten
   if (!r->finished) return fail(HB_ERR_DEVICE, "server error: %s", S->error.c_str());
   if (outputs) std::memcpy(outputs, r->out.data(), r->out.size() * 8);
   if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+  if (core_traces && r->traced) {
+    // schedule order -> gate order; widen BabyBear words
+    const size_t words = r->trace_gate_bytes / (r->ek->p.is_goldilocks ? 8 : 4);
+    for (uint32_t s = 0; s < r->n_gates; s++) {
+      const uint32_t g = r->idx[3 * (size_t)s + 2] - r->n_inputs;
+      const uint8_t* src = r->trace_host.data() + (size_t)s * r->trace_gate_bytes;
+      uint64_t* dst = core_traces + (size_t)g * words;
+      if (r->ek->p.is_goldilocks) {
+        std::memcpy(dst, src, words * 8);
+      } else {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(src);
+        for (size_t k = 0; k < words; k++) dst[k] = w[k];
+      }
+    }
+  }
   S->reqs.erase(it);
   return HB_OK;
+}
+
+int hb_server_wait(hb_server_t* S, uint64_t id, uint64_t* outputs, double* latency_ms) {
+  return hb_server_wait_traced(S, id, outputs, nullptr, latency_ms);
 }
 
 int hb_server_stats(hb_server_t* S, hb_server_stats_t* out) {

@@ -87,7 +87,8 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, evaluator, circuit: Circuit, inputs: np.ndarray) -> int:
+    def submit(self, evaluator, circuit: Circuit, inputs: np.ndarray, traced: bool = False) -> int:
+        """traced: run with the vfhe semantics and keep every gate's core trace."""
         inputs = np.ascontiguousarray(inputs, np.uint64)
         arrays = [np.ascontiguousarray(a) for a in
                   (circuit.in0.astype(np.uint32), circuit.in1.astype(np.uint32),
@@ -96,17 +97,24 @@ class Server:
                       arrays[0].ctypes.data, arrays[1].ctypes.data, arrays[2].ctypes.data,
                       arrays[3].ctypes.data)
         rid = C.c_uint64()
-        check(lib().hb_server_submit(self._h, evaluator._h, C.byref(c), ptr(inputs), C.byref(rid)))
+        check(lib().hb_server_submit_traced(self._h, evaluator._h, C.byref(c), ptr(inputs),
+                                            int(traced), C.byref(rid)))
+        P = evaluator.params
         with self._lock:
-            self._keep[rid.value] = (int(circuit.outputs.size), evaluator.params.n + 1)
+            self._keep[rid.value] = (int(circuit.outputs.size), P.n + 1, circuit.n_gates, P.N,
+                                     traced)
         return rid.value
 
-    def wait(self, rid: int) -> tuple[np.ndarray, float]:
+    def wait(self, rid: int):
+        """(outputs, latency_ms), plus the core traces (n_gates, n+1, 2, N) for traced requests."""
         with self._lock:
-            n_out, n1 = self._keep.pop(rid)
+            n_out, n1, n_gates, N, traced = self._keep.pop(rid)
         out = np.zeros((n_out, n1), dtype=np.uint64)
         lat = C.c_double()
-        check(lib().hb_server_wait(self._h, rid, ptr(out), C.byref(lat)))
+        core = np.zeros((n_gates, n1, 2, N), dtype=np.uint64) if traced else None
+        check(lib().hb_server_wait_traced(self._h, rid, ptr(out), ptr(core), C.byref(lat)))
+        if traced:
+            return out, lat.value, core
         return out, lat.value
 
     def done(self, rid: int) -> bool:

@@ -206,10 +206,19 @@ int hb_server_destroy(hb_server_t* server);
  * never fit the budget. */
 int hb_server_submit(hb_server_t* server, hb_ek_t* ek, const hb_circuit_t* circuit,
                      const uint64_t* inputs, uint64_t* request_id);
+/* As hb_server_submit; traced != 0 runs the circuit with the traced (vfhe) semantics and
+ * keeps each gate's core trace ((n+1)*2N words, see hb_gate_batch_traced) in HBM, counted
+ * against the budget, until hb_server_wait_traced returns it. */
+int hb_server_submit_traced(hb_server_t* server, hb_ek_t* ek, const hb_circuit_t* circuit,
+                            const uint64_t* inputs, int traced, uint64_t* request_id);
 int hb_server_poll(hb_server_t* server, uint64_t request_id, int* done);
 /* Blocks until done; outputs: n_outputs ciphertexts; latency = submit → done (ms). */
 int hb_server_wait(hb_server_t* server, uint64_t request_id, uint64_t* outputs,
                    double* latency_ms);
+/* As hb_server_wait, plus core_traces: n_gates * (n+1) * 2N words in gate order (traced
+ * requests only; NULL skips). */
+int hb_server_wait_traced(hb_server_t* server, uint64_t request_id, uint64_t* outputs,
+                          uint64_t* core_traces, double* latency_ms);
 int hb_server_stats(hb_server_t* server, hb_server_stats_t* out);
 
 /* Standalone negacyclic NTT over `count` polynomials of N = 2^log_n words (device

@@ -95,6 +95,42 @@ def test_oversized_circuit_is_rejected(hb, clients):
     srv.close()
 
 
+def test_traced_request_traces_match_oracle(hb, orc, keyset):
+    """A traced circuit served next to an untraced one: every gate's emitted core trace,
+    expanded, equals the oracle's traced records for that gate (rounds 0, 363, 727 and the
+    tail), and the outputs equal the oracle's traced evaluation."""
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    P, sk, bsk, ksk = keyset(hb.SET_BABYBEAR)
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+    enc = hb.Encryptor(sk)
+    rng = stream(54, "traced")
+    circ = random_nand_circuit(rng, 2, 3)
+    bits = rng.integers(0, 2, size=3)
+    cts = enc.encrypt_many(bits)
+    other = random_nand_circuit(rng, 4, 6)
+    srv = Server(max_gates_per_launch=64)
+    r_other = srv.submit(ev, other, enc.encrypt_many(rng.integers(0, 2, size=6)))
+    r = srv.submit(ev, circ, cts, traced=True)
+    out, _, core = srv.wait(r)
+    srv.wait(r_other)
+    srv.close()
+    ok = orc.OracleKeys(hb.SET_BABYBEAR, bsk, ksk)
+    lut = orc.build_lut(orc.params(hb.SET_BABYBEAR), 0)
+    rounds = [0, 363, 727]
+    wires = list(cts)
+    for g in range(circ.n_gates):
+        a, b = wires[circ.in0[g]], wires[circ.in1[g]]
+        c = ((a.astype(object) + b.astype(object)) % P.q).astype(np.uint64)
+        ref_out, ref_rec, ref_tail = ok.bootstrap_traced(c, lut, rounds)
+        rec, tail = ev.expand_trace("nand", a, b, core[g], rounds)
+        assert np.array_equal(rec, ref_rec), g
+        assert np.array_equal(tail, ref_tail), g
+        wires.append(ref_out)
+    assert np.array_equal(out, np.stack([wires[o] for o in circ.outputs]))
+    dec = hb.Decryptor(sk)
+    assert [dec.decrypt(hb.LWECiphertext(out[k])) for k in range(3)] == list(circ.eval_plain(bits))
+
+
 def test_server_matches_oracle_gate_by_gate(hb, orc, keyset):
     from hasteboots_amd.serving import Server, random_nand_circuit
     P, sk, bsk, ksk = keyset(hb.SET_BABYBEAR)
