@@ -67,6 +67,11 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="gates compared with the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-rates", default="2",
+                    help="comma-separated circuit arrival rates (1/s) for the serving leg; "
+                         "empty to skip")
+    ap.add_argument("--serving-seconds", type=float, default=8.0)
+    ap.add_argument("--serving-clients", type=int, default=8)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -160,6 +165,36 @@ def main() -> int:
                  "p99_ms": round(float(np.percentile(lat, 99)), 3), "gates": args.plain}
         log(f"plain: {plain_value:.2f} bootstraps/s, p50 {np.percentile(lat, 50):.1f} ms")
 
+    serving = None
+    rates = [float(r) for r in args.serving_rates.split(",") if r.strip()]
+    if rates:
+        from hasteboots_amd.serving import Server, run_load
+        t = time.time()
+        clients = [(ev, enc, dec)]
+        for seed in range(1, args.serving_clients):
+            sk_c = hb.KeyGen.generate_secret_key(P, seed=seed)
+            bsk_c, ksk_c = sk_c.generate_keys_host()
+            clients.append((hb.Evaluator(keys=(bsk_c, ksk_c), params=P, device=0),
+                            hb.Encryptor(sk_c), hb.Decryptor(sk_c)))
+        log(f"serving: {len(clients)} clients with their own keys ready in {time.time() - t:.1f}s")
+        sweep = []
+        for rate in rates:
+            srv = Server(device=0, max_gates_per_launch=2048)
+            res = run_load(srv, clients, rate, args.serving_seconds,
+                           np.random.default_rng([7, int(rate * 1000)]))
+            st = srv.stats()
+            srv.close()
+            res.update({"gate_p50_ms": round(st["gate_p50_ms"], 1),
+                        "gate_p99_ms": round(st["gate_p99_ms"], 1),
+                        "launches": st["launches"], "kernel_busy_ms": round(st["busy_ms"], 1)})
+            log(f"serving rate {rate}/s: {res}")
+            sweep.append(res)
+        serving = {"clients": len(clients), "parameter_set": SET_NAMES[set_id],
+                   "circuit_depth": [1, 64], "circuit_width": [2, 64], "arrivals": "poisson",
+                   "latency": "circuit = submit to outputs on host; gate = inputs ready to "
+                              "launch done", "sweep": sweep}
+        del clients
+
     wb = 8 if P.is_goldilocks else 4
     key_bytes = (bsk.size + ksk.size) * wb
     io_bytes = G * 3 * n1 * 8
@@ -222,6 +257,7 @@ def main() -> int:
                  "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1)},
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
                     "oracle_gates": args.check},
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

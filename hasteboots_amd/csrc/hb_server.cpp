@@ -11,6 +11,7 @@
 // There is no reference counterpart: the reference evaluates gates one call at a time.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -99,6 +100,7 @@ struct Request {
   std::vector<uint8_t> trace_host;
   std::vector<uint64_t> out;
   double t_submit = 0, t_admit = 0, t_done = 0;
+  double t_ready = 0;  // when the gates of the next run got their inputs
 };
 
 struct Slot {
@@ -133,6 +135,8 @@ struct hb_server {
   hipStream_t copy = nullptr;
   hb_server_stats_t st{};
   std::string error;
+  // gate latency samples: (ms from inputs ready to launch completion, gates)
+  std::vector<std::pair<float, uint32_t>> gate_lat;
 };
 
 namespace {
@@ -156,6 +160,7 @@ int admit(hb_server* S) {
     HIP_TRY(hipStreamSynchronize(S->copy));
     std::vector<uint64_t>().swap(r->inputs);
     r->t_admit = now_ms();
+    r->t_ready = r->t_admit;
     S->active.push_back(r);
     S->queue.pop_front();
     S->st.admitted++;
@@ -190,10 +195,13 @@ int complete(hb_server* S, Slot& sl) {
   float ms = 0;
   if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) S->st.busy_ms += ms;
   S->st.gates_done += sl.gates;
+  const double t = now_ms();
   for (auto& w : sl.work) {
     Request* r = w.first;
     r->completed += w.second;
     r->in_flight = false;
+    S->gate_lat.emplace_back((float)(t - r->t_ready), w.second);
+    r->t_ready = t;
     if (r->completed == r->n_gates) {
       int rc = finish(S, r);
       if (rc) return rc;
@@ -500,6 +508,23 @@ int hb_server_stats(hb_server_t* S, hb_server_stats_t* out) {
   out->bytes_in_use = S->arena.used();
   out->queued = S->queue.size();
   out->active = S->active.size();
+  // gate-weighted percentiles of the gate latency
+  std::vector<std::pair<float, uint32_t>> v = S->gate_lat;
+  std::sort(v.begin(), v.end());
+  uint64_t total = 0;
+  for (auto& x : v) total += x.second;
+  auto pct = [&](double q) -> double {
+    if (total == 0) return 0.0;
+    const uint64_t target = (uint64_t)std::ceil(q * (double)total);
+    uint64_t seen = 0;
+    for (auto& x : v) {
+      seen += x.second;
+      if (seen >= target) return x.first;
+    }
+    return v.back().first;
+  };
+  out->gate_p50_ms = pct(0.50);
+  out->gate_p99_ms = pct(0.99);
   return HB_OK;
 }
 

@@ -195,6 +195,9 @@ typedef struct {
   uint64_t queued, active;
   uint64_t bytes_in_use, budget;
   double busy_ms; /* summed event-timed kernel duration */
+  /* gate latency = from a gate's inputs being ready (admission or previous layer done) to
+   * the completion of the launch that ran it; gate-weighted percentiles since creation */
+  double gate_p50_ms, gate_p99_ms;
 } hb_server_stats_t;
 
 /* max_gates_per_launch: gate slots per launch; hbm_budget_bytes: arena size (0 = half of
