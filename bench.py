@@ -15,6 +15,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -46,6 +47,16 @@ def modmuls_per_gate(P) -> int:
     return P.n * per_round + ks
 
 
+def profiled_traffic(set_name: str, gates: int):
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE passes (scaled by a
+    copy kernel of known size), recorded in profiles/ for this workload; None if absent."""
+    p = ROOT / "profiles" / f"bootstrap_kernel_{set_name}_{gates}_hbm_traffic.md"
+    if not p.exists():
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", p.read_text())
+    return (int(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -72,6 +83,9 @@ def main() -> int:
                          "empty to skip")
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--serving-clients", type=int, default=8)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before the batched leg, copy a device buffer of this size once "
+                         "(a reference kernel for scaling HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -109,6 +123,17 @@ def main() -> int:
         check(L.hb_dev_alloc(arr.nbytes, C.byref(p)))
         check(L.hb_memcpy_h2d(p, arr.ctypes.data, arr.nbytes))
         return p
+
+    if args.copy_probe_mib > 0:
+        import torch
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nb} bytes device to device")
+        del src, dst
 
     d0, d1 = dev(c0), dev(c1)
     dout = dev(np.zeros_like(c0))
@@ -202,6 +227,7 @@ def main() -> int:
     kms = float(np.mean(kernel_ms))
     achieved = traffic_alg / (kms * 1e-3)
     mm = modmuls_per_gate(P)
+    traffic, traffic_src = profiled_traffic(args.set, G)
 
     cpu = None
     if not args.no_cpu:
@@ -251,7 +277,8 @@ def main() -> int:
         "kernel_ms_per_launch": round(kms, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 3),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": traffic_alg},
         "valu": {"modmul_per_gate": mm,
                  "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1)},
