@@ -1,0 +1,392 @@
+"""Generates hb_gl_asm.inc: hand-scheduled gfx950 VALU sequences for Goldilocks.
+
+Run `python gen_gl_asm.py` in this directory after editing; a CPU test checks that the
+committed hb_gl_asm.inc matches this generator's output.
+
+Arithmetic (P = 2^64 - 2^32 + 1, EPS = 2^32 - 1, words are 64-bit pairs of 32-bit halves):
+  montmul(y, w): y in [0, 2^64), w < P stored times 2^64 (Montgomery constant form):
+    (lo, hi) = y * w from four 32x32 products chained through v_mad_u64_u32, then the
+    Goldilocks Montgomery reduction
+        a = lo + (lo << 32) mod 2^64 (carry e);  b = a - (a >> 32) - e;
+        r = hi - b (borrow c);  r -= c ? EPS : 0
+    which returns y * (w / 2^64) mod P in [0, P) whenever hi < P (hi < w < P here).
+  CT butterfly (forward):  t = montmul(y, w);  x' = x + t;  y' = x - t.
+    t < P, so each of the add and the subtract needs one conditional EPS correction;
+    x' and y' are in [0, 2^64).
+  GS butterfly (inverse), x and y canonical:  x' = canon(x + y);  y' = montmul(x - y mod P, w).
+  MAC:  p' = p + montmul(x, k) with p in [0, 2^64); one EPS correction.
+Conditional +-EPS uses v_mad_i64_i32(m, 0x55555555, v) with m in {3, -3, 0}
+(3 * 0x55555555 = EPS), or v_mad_u64_u32(m, 1, v) with m in {EPS, 0}.
+
+Scheduling: two independent chains are interleaved by a list scheduler. gfx950 needs
+two wait states between a VALU write of an SGPR (carry out / compare) and a VALU read
+of it as carry-in or select mask; the scheduler fills them with the other chain's work
+and emits s_nop only when nothing is ready. Temporaries live in fixed VGPRs
+v232..v255 (clobbered); inputs and outputs are compiler-allocated operands, and every
+output is written after the last read of every input, so outputs may share registers
+with inputs.
+"""
+from __future__ import annotations
+
+import os
+import re
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "hb_gl_asm.inc")
+
+TEMP_BASE = 232  # v232..v255: 12 per chain
+
+
+class Chain:
+    """Instruction list of one butterfly/MAC chain with symbolic registers.
+
+    Symbolic names: 'A','U','V','W','Z' are fixed pairs (".lo"/".hi" halves), 'T1','T2'
+    fixed singles, 'S1','S2' SGPR-pair operands, '%name' compiler operands, 'D' dummy
+    SGPR pair. Each instruction is (template, defs, uses).
+    """
+
+    def __init__(self, tag: str, base: int):
+        self.tag = tag
+        self.base = base
+        self.ins: list[tuple[str, list[str], list[str]]] = []
+
+    def reg(self, name: str) -> str:
+        pairs = {"A": 0, "U": 2, "V": 4, "W": 6, "Z": 8}
+        if name in ("T1", "T2"):
+            return f"v{self.base + 10 + (0 if name == 'T1' else 1)}"
+        m = re.fullmatch(r"([AUVWZ])(\.lo|\.hi)?", name)
+        if m:
+            b = self.base + pairs[m.group(1)]
+            if m.group(2) == ".lo":
+                return f"v{b}"
+            if m.group(2) == ".hi":
+                return f"v{b + 1}"
+            return f"v[{b}:{b + 1}]"
+        raise KeyError(name)
+
+    def add(self, text: str, defs: list[str], uses: list[str]):
+        self.ins.append((text, defs, uses))
+
+
+def is_sgpr(sym: str) -> bool:
+    return sym.startswith("S") or sym == "D" or sym.startswith("%s")
+
+
+def halves(sym: str) -> list[str]:
+    """Register atoms of a symbol, for dependence tracking (pairs -> both halves)."""
+    if re.fullmatch(r"[AUVWZ]", sym):
+        return [sym + ".lo", sym + ".hi"]
+    return [sym]
+
+
+def montmul(c: Chain, y0: str, y1: str, w0: str, w1: str, dst: str = "A"):
+    """dst (default pair A) <- montmul(y, w); uses A, U, V, W, Z, T1, T2, S1, S2."""
+    c.add(f"v_mad_u64_u32 {{A}}, {{D}}, {y0}, {w0}, 0", ["A"], [y0, w0])
+    c.add("v_mov_b32 {Z.lo}, {A.hi}", ["Z.lo"], ["A.hi"])
+    c.add(f"v_mad_u64_u32 {{U}}, {{D}}, {y1}, {w0}, {{Z}}", ["U"], [y1, w0, "Z"])
+    c.add("v_mov_b32 {Z.lo}, {U.lo}", ["Z.lo"], ["U.lo"])
+    c.add(f"v_mad_u64_u32 {{V}}, {{D}}, {y0}, {w1}, {{Z}}", ["V"], [y0, w1, "Z"])
+    c.add("v_mov_b32 {Z.lo}, {U.hi}", ["Z.lo"], ["U.hi"])
+    c.add(f"v_mad_u64_u32 {{W}}, {{D}}, {y1}, {w1}, {{Z}}", ["W"], [y1, w1, "Z"])
+    c.add("v_mad_u64_u32 {W}, {D}, {V.hi}, 1, {W}", ["W"], ["V.hi", "W"])
+    # lo = (A.lo, V.lo), hi = (W.lo, W.hi)
+    c.add("v_add_co_u32 {T1}, {S1}, {V.lo}, {A.lo}", ["T1", "S1"], ["V.lo", "A.lo"])
+    c.add("v_subb_co_u32 {T2}, {S2}, {A.lo}, {T1}, {S1}", ["T2", "S2"], ["A.lo", "T1", "S1"])
+    c.add("v_subb_co_u32 {T1}, {S2}, {T1}, 0, {S2}", ["T1", "S2"], ["T1", "S2"])
+    c.add("v_sub_co_u32 {A.lo}, {S1}, {W.lo}, {T2}", ["A.lo", "S1"], ["W.lo", "T2"])
+    c.add("v_subb_co_u32 {A.hi}, {S1}, {W.hi}, {T1}, {S1}", ["A.hi", "S1"], ["W.hi", "T1", "S1"])
+    c.add("v_cndmask_b32 {T2}, 0, -3, {S1}", ["T2"], ["S1"])
+    c.add(f"v_mad_i64_i32 {{{dst}}}, {{D}}, {{T2}}, {{%k55}}, {{A}}", [dst], ["T2", "%k55", "A"])
+
+
+def ct_chain(tag: str, base: int, wkind: str) -> Chain:
+    """x' = x + montmul(y, w), y' = x - montmul(y, w)."""
+    c = Chain(tag, base)
+    x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
+    w0, w1 = f"%{wkind}w{tag}0", f"%{wkind}w{tag}1"
+    montmul(c, y0, y1, w0, w1)
+    c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {{A.lo}}", ["U.lo", "S1"], [x0, "A.lo"])
+    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {{A.lo}}", ["V.lo", "S2"], [x0, "A.lo"])
+    c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {{A.hi}}, {{S1}}", ["U.hi", "S1"], [x1, "A.hi", "S1"])
+    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {{A.hi}}, {{S2}}", ["V.hi", "S2"], [x1, "A.hi", "S2"])
+    c.add("v_cndmask_b32 {T1}, 0, -1, {S1}", ["T1"], ["S1"])
+    c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
+    c.add(f"v_mad_u64_u32 {{%X{tag}}}, {{D}}, {{T1}}, 1, {{U}}", [f"%X{tag}"], ["T1", "U"])
+    c.add(f"v_mad_i64_i32 {{%Y{tag}}}, {{D}}, {{T2}}, {{%k55}}, {{V}}", [f"%Y{tag}"],
+          ["T2", "%k55", "V"])
+    return c
+
+
+def gs_chain(tag: str, base: int, wkind: str) -> Chain:
+    """x, y canonical: x' = canon(x + y) (two halves out), y' = montmul(x - y, w)."""
+    c = Chain(tag, base)
+    x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
+    w0, w1 = f"%{wkind}w{tag}0", f"%{wkind}w{tag}1"
+    # d = x - y mod P, canonical, into T1/T2 (via pair V)
+    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {y0}", ["V.lo", "S2"], [x0, y0])
+    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {y1}, {{S2}}", ["V.hi", "S2"], [x1, y1, "S2"])
+    c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
+    c.add("v_mad_i64_i32 {V}, {D}, {T2}, {%k55}, {V}", ["V"], ["T2", "%k55", "V"])
+    c.add("v_mov_b32 {T1}, {V.lo}", ["T1"], ["V.lo"])
+    c.add("v_mov_b32 {T2}, {V.hi}", ["T2"], ["V.hi"])
+    # y' = montmul(d, w) straight into the output pair; montmul rewrites T1/T2 only
+    # after its last multiply has read them
+    sub = Chain(tag, base)
+    montmul(sub, "{T1}", "{T2}", w0, w1, dst=f"%Y{tag}")
+    for text, defs, uses in sub.ins:
+        c.add(text, defs, ["T1" if u == "{T1}" else "T2" if u == "{T2}" else u for u in uses])
+    # x' = canon(x + y): s = x + y (carry S1); q = s + EPS (carry S2); q if S1 | S2
+    c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {y0}", ["U.lo", "S1"], [x0, y0])
+    c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {y1}, {{S1}}", ["U.hi", "S1"], [x1, y1, "S1"])
+    c.add("v_mad_u64_u32 {W}, {S2}, -1, 1, {U}", ["W", "S2"], ["U"])
+    c.add("s_or_b64 {S1}, {S1}, {S2}", ["S1"], ["S1", "S2"])
+    c.add(f"v_cndmask_b32 {{%X{tag}lo}}, {{U.lo}}, {{W.lo}}, {{S1}}", [f"%X{tag}lo"],
+          ["U.lo", "W.lo", "S1"])
+    c.add(f"v_cndmask_b32 {{%X{tag}hi}}, {{U.hi}}, {{W.hi}}, {{S1}}", [f"%X{tag}hi"],
+          ["U.hi", "W.hi", "S1"])
+    return c
+
+
+def mac_chain(tag: str, base: int) -> Chain:
+    """p' = p + montmul(x, k); x shared by both chains of a block."""
+    c = Chain(tag, base)
+    montmul(c, "%x0", "%x1", f"%k{tag}0", f"%k{tag}1")
+    p0, p1 = f"%p{tag}0", f"%p{tag}1"
+    c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {p0}, {{A.lo}}", ["U.lo", "S1"], [p0, "A.lo"])
+    c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {p1}, {{A.hi}}, {{S1}}", ["U.hi", "S1"], [p1, "A.hi", "S1"])
+    c.add("v_cndmask_b32 {T1}, 0, -1, {S1}", ["T1"], ["S1"])
+    c.add(f"v_mad_u64_u32 {{%P{tag}}}, {{D}}, {{T1}}, 1, {{U}}", [f"%P{tag}"], ["T1", "U"])
+    return c
+
+
+def schedule(chains: list[Chain]) -> list[str]:
+    """List-schedule the chains; returns instruction lines with physical registers."""
+    # nodes: (chain index, instr index)
+    nodes = []
+    for ci, c in enumerate(chains):
+        for ii, (text, defs, uses) in enumerate(c.ins):
+            nodes.append((ci, ii))
+    # per-node dependence list: (pred node, min distance)
+    deps = {n: [] for n in nodes}
+
+    def atoms(ci, syms):
+        out = []
+        for s in syms:
+            s = s.strip("{}")
+            if s.startswith("%") or s == "D":
+                out.append(s if s.startswith("%") else "D")
+            else:
+                out += [f"{ci}:{a}" for a in halves(s)]
+        return out
+
+    for ci, c in enumerate(chains):
+        for j, (tj, dj, uj) in enumerate(c.ins):
+            aj_def, aj_use = atoms(ci, dj), atoms(ci, uj)
+            for i in range(j):
+                ti, di, ui = c.ins[i]
+                ai_def, ai_use = atoms(ci, di), atoms(ci, ui)
+                for r in set(ai_def) & set(aj_use):
+                    if r == "D":
+                        continue
+                    sg = r.split(":")[-1].startswith("S")
+                    valu_writer = not ti.startswith("s_")
+                    valu_reader = not tj.startswith("s_")
+                    dist = 3 if (sg and valu_writer and valu_reader) else 1
+                    deps[(ci, j)].append(((ci, i), dist))
+                for r in (set(ai_use) & set(aj_def)) | (set(ai_def) & set(aj_def)):
+                    if r == "D":
+                        continue
+                    deps[(ci, j)].append(((ci, i), 1))
+    # outputs (%X/%Y/%P defs) only after every read of any input operand
+    in_readers = [n for n in nodes
+                  if any(u.startswith("%") and u != "%k55"
+                         for u in chains[n[0]].ins[n[1]][2])]
+    for n in nodes:
+        d = chains[n[0]].ins[n[1]][1]
+        if any(x.startswith("%") for x in d):
+            for r in in_readers:
+                if r != n:
+                    deps[n].append((r, 1))
+    # critical-path priority
+    succ = {n: [] for n in nodes}
+    for n, ds in deps.items():
+        for p, d in ds:
+            succ[p].append((n, d))
+    prio = {}
+
+    def height(n):
+        if n in prio:
+            return prio[n]
+        h = 1 + max((d + height(s) - 1 for s, d in succ[n]), default=0)
+        prio[n] = h
+        return h
+
+    for n in nodes:
+        height(n)
+    slot = {}
+    out = []
+    t = 0
+    remaining = set(nodes)
+    while remaining:
+        ready = [n for n in remaining
+                 if all(p in slot and slot[p] + d <= t for p, d in deps[n])]
+        if not ready:
+            out.append(("s_nop 0", None))
+            t += 1
+            continue
+        n = max(ready, key=lambda n: (prio[n], -n[1], -n[0]))
+        slot[n] = t
+        remaining.discard(n)
+        out.append((chains[n[0]].ins[n[1]][0], n[0]))
+        t += 1
+    lines = []
+    for text, ci in out:
+        if ci is None:
+            lines.append(text)
+            continue
+        c = chains[ci]
+
+        def sub(m):
+            s = m.group(1)
+            if s.startswith("%"):
+                return "{" + s + "}"
+            if s == "D":
+                return "{%dummy}"
+            if s in ("S1", "S2"):
+                return "{%" + f"s{s[1]}{c.tag}" + "}"
+            return c.reg(s)
+
+        lines.append(re.sub(r"\{([^}]+)\}", sub, text))
+    return lines
+
+
+def verify(lines: list[str], n_out: int, vgpr_inputs: set[int]) -> None:
+    """Static checks of one emitted block (operands already numbered):
+    * a VALU write of an SGPR operand and a later VALU read of it as carry-in or mask
+      are at least 3 instructions apart (two wait states on gfx950);
+    * no output operand (%0..%n_out-1) is written before the last read of a VGPR input
+      operand, so outputs may share registers with inputs;
+    * every fixed temporary is written in the block before it is read."""
+    def ops(ln):
+        mnem, _, rest = ln.partition(" ")
+        args = [a.strip() for a in rest.split(",")] if rest else []
+        return mnem, args
+
+    sgpr_write = {}
+    last_in_read = -1
+    first_out_write = None
+    written = set()
+    for t, ln in enumerate(lines):
+        mnem, args = ops(ln)
+        if mnem == "s_nop":
+            continue
+        dsts, srcs = [], args
+        if mnem.startswith("v_") or mnem.startswith("s_"):
+            if mnem in ("v_add_co_u32", "v_sub_co_u32", "v_addc_co_u32", "v_subb_co_u32",
+                        "v_mad_u64_u32", "v_mad_i64_i32"):
+                dsts, srcs = args[:2], args[2:]
+            else:
+                dsts, srcs = args[:1], args[1:]
+        for a in srcs:
+            m = re.fullmatch(r"%(\d+)", a)
+            if m:
+                k = int(m.group(1))
+                if k in vgpr_inputs:
+                    last_in_read = t
+                if a in sgpr_write and mnem.startswith("v_"):
+                    assert t - sgpr_write[a] >= 3, f"SGPR hazard at {t}: {ln}"
+            vm = re.fullmatch(r"v\[?(\d+)(?::(\d+)\])?", a)
+            if vm:
+                lo = int(vm.group(1))
+                hi = int(vm.group(2) or lo)
+                for r in range(lo, hi + 1):
+                    assert r in written, f"read of unwritten v{r} at {t}: {ln}"
+        for a in dsts:
+            m = re.fullmatch(r"%(\d+)", a)
+            if m:
+                k = int(m.group(1))
+                if k < n_out and first_out_write is None:
+                    first_out_write = t
+                if mnem.startswith("v_"):
+                    sgpr_write[a] = t
+                else:
+                    sgpr_write.pop(a, None)
+            vm = re.fullmatch(r"v\[?(\d+)(?::(\d+)\])?", a)
+            if vm:
+                lo = int(vm.group(1))
+                hi = int(vm.group(2) or lo)
+                written.update(range(lo, hi + 1))
+    assert first_out_write is None or first_out_write > last_in_read, "output before input read"
+
+
+def emit(name: str, chains: list[Chain], outputs: list[tuple[str, str, str]],
+         inputs: list[tuple[str, str, str]], doc: str) -> str:
+    """outputs/inputs: (symbol, constraint, C expression)."""
+    lines = [f"v_mov_b32 v{c.base + 9}, 0" for c in chains] + schedule(chains)
+    sg = []
+    for c in chains:
+        sg += [(f"s1{c.tag}", "=&s", f"s1{c.tag}"), (f"s2{c.tag}", "=&s", f"s2{c.tag}")]
+    ops = outputs + [("dummy", "=&s", "dummy")] + sg + inputs
+    index = {sym: i for i, (sym, _, _) in enumerate(ops)}
+    body = []
+    for ln in lines:
+        ln = re.sub(r"\{%([A-Za-z0-9]+)\}", lambda m: f"%{index[m.group(1)]}", ln)
+        ln = re.sub(r"%([A-Za-z][A-Za-z0-9]*)", lambda m: f"%{index[m.group(1)]}", ln)
+        body.append(f'      "{ln}\\n"')
+    outs = ", ".join(f'"{c}"({e})' for _, c, e in outputs + [("dummy", "=&s", "dummy")] + sg)
+    ins = ", ".join(f'"{c}"({e})' for _, c, e in inputs)
+    clob = ", ".join(f'"v{r}"' for r in range(TEMP_BASE, 256))
+    numbered = [re.sub(r"%([A-Za-z][A-Za-z0-9]*)", lambda m: f"%{index[m.group(1)]}",
+                       re.sub(r"\{%([A-Za-z0-9]+)\}", lambda m: f"%{index[m.group(1)]}", ln))
+                for ln in lines]
+    verify(numbered, len(outputs), {index[sym] for sym, c, _ in inputs if c == "v"})
+    n_nop = sum(1 for ln in lines if ln.startswith("s_nop"))
+    return (f"// {doc}\n// {len(lines)} instructions, {n_nop} of them s_nop, for "
+            f"{len(chains)} chains.\n"
+            f"#define HB_GL_{name.upper()}_ASM \\\n"
+            "  asm( \\\n" + " \\\n".join(body) + " \\\n"
+            f"      : {outs} \\\n      : {ins} \\\n      : {clob}, \"scc\")\n")
+
+
+def main():
+    parts = ["// Generated by gen_gl_asm.py — do not edit.\n"
+             "// Each macro expects the variables its operand lists name to be in scope.\n"]
+    for wkind, wc in (("v", "v"), ("s", "s")):
+        a, b = ct_chain("a", TEMP_BASE, wkind), ct_chain("b", TEMP_BASE + 12, wkind)
+        outputs = [("Xa", "=v", "Xa"), ("Ya", "=v", "Ya"), ("Xb", "=v", "Xb"), ("Yb", "=v", "Yb")]
+        inputs = []
+        for t in "ab":
+            inputs += [(f"x{t}0", "v", f"(u32)x{t}"), (f"x{t}1", "v", f"(u32)(x{t} >> 32)"),
+                       (f"y{t}0", "v", f"(u32)y{t}"), (f"y{t}1", "v", f"(u32)(y{t} >> 32)"),
+                       (f"{wkind}w{t}0", wc, f"(u32)w{t}"),
+                       (f"{wkind}w{t}1", wc, f"(u32)(w{t} >> 32)")]
+        inputs.append(("k55", "s", "0x55555555u"))
+        parts.append(emit(f"ct2_{wkind}", [a, b], outputs, inputs,
+                          f"Two CT butterflies (x, y) <- (x + w*y, x - w*y), twiddles in "
+                          f"{'SGPRs' if wkind == 's' else 'VGPRs'}."))
+        a, b = gs_chain("a", TEMP_BASE, wkind), gs_chain("b", TEMP_BASE + 12, wkind)
+        outputs = [("Xalo", "=v", "Xalo"), ("Xahi", "=v", "Xahi"), ("Ya", "=v", "Ya"),
+                   ("Xblo", "=v", "Xblo"), ("Xbhi", "=v", "Xbhi"), ("Yb", "=v", "Yb")]
+        parts.append(emit(f"gs2_{wkind}", [a, b], outputs, inputs,
+                          f"Two GS butterflies (x, y) <- (x + y, (x - y)*w) on canonical "
+                          f"inputs, twiddles in {'SGPRs' if wkind == 's' else 'VGPRs'}."))
+    a, b = mac_chain("a", TEMP_BASE), mac_chain("b", TEMP_BASE + 12)
+    outputs = [("Pa", "=v", "Pa"), ("Pb", "=v", "Pb")]
+    inputs = [("x0", "v", "(u32)x"), ("x1", "v", "(u32)(x >> 32)")]
+    for t in "ab":
+        inputs += [(f"k{t}0", "v", f"(u32)k{t}"), (f"k{t}1", "v", f"(u32)(k{t} >> 32)"),
+                   (f"p{t}0", "v", f"(u32)p{t}"), (f"p{t}1", "v", f"(u32)(p{t} >> 32)")]
+    inputs.append(("k55", "s", "0x55555555u"))
+    parts.append(emit("mac2", [a, b], outputs, inputs,
+                      "(pa, pb) += (x * ka, x * kb), keys in Montgomery constant form."))
+    text = "\n".join(parts)
+    if len(sys.argv) > 1 and sys.argv[1] == "--check":
+        return 0 if open(OUT).read() == text else 1
+    with open(OUT, "w") as f:
+        f.write(text)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

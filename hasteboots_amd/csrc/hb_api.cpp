@@ -46,7 +46,7 @@ namespace {
 
 uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
   // scale: a canonical multiplier folded into the constant (e.g. N^-1).
-  if (p.is_goldilocks) return Gl::canon(Gl::mul(v, scale));
+  if (p.is_goldilocks) return Gl::to_mont(Gl::canon(Gl::mul(v, scale)));
   uint32_t x = (uint32_t)((v * scale) % p.q);
   return Bb::to_mont(x);
 }
@@ -80,7 +80,9 @@ int hb::ensure_engine(int device) {
         void* d = nullptr;
         HIP_TRY(hipMalloc(&d, bytes));
         if (f) {
-          HIP_TRY(hipMemcpy(d, src.data(), bytes, hipMemcpyHostToDevice));
+          std::vector<uint64_t> w(src.size());
+          for (size_t k = 0; k < src.size(); k++) w[k] = Gl::to_mont(src[k]);
+          HIP_TRY(hipMemcpy(d, w.data(), bytes, hipMemcpyHostToDevice));
         } else {
           std::vector<uint32_t> w(src.size());
           for (size_t k = 0; k < src.size(); k++) w[k] = Bb::to_mont((uint32_t)src[k]);
@@ -88,7 +90,7 @@ int hb::ensure_engine(int device) {
         }
         E.tables.tw[f][lg - 10][inv] = d;
       }
-      E.tables.ninv[f][lg - 10] = f ? ntt.n_inv : Bb::to_mont((uint32_t)ntt.n_inv);
+      E.tables.ninv[f][lg - 10] = f ? Gl::to_mont(ntt.n_inv) : Bb::to_mont((uint32_t)ntt.n_inv);
     }
   }
   E.tables.ready = true;

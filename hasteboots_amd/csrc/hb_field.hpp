@@ -95,6 +95,25 @@ struct Gl {
     mul_wide(a, b, lo, hi);
     return reduce128(lo, hi);
   }
+  // Montgomery reduction with R = 2^64: for x = lo + hi * 2^64 with hi < P returns
+  // x * 2^-64 mod P in [0, P). m = lo * P^-1 mod 2^64 = lo * (1 + 2^32); b = m*P >> 64
+  // is formed from m without a multiply; r = hi - b, plus P on borrow.
+  HB_HD static u64 mont_reduce(u64 lo, u64 hi) {
+    u64 a = lo + (lo << 32);
+    u64 e = a < lo ? 1 : 0;
+    u64 b = a - (a >> 32) - e;
+    u64 r = hi - b;
+    return hi < b ? r - EPS : r;
+  }
+  // a * c * 2^-64 mod P: device constants (twiddles, keys) are stored as c * 2^64 mod P,
+  // so mont_mul(x, c * 2^64) = x * c with x in plain form. Canonical output.
+  HB_HD static u64 mont_mul(u64 a, u64 c) {
+    u64 lo, hi;
+    mul_wide(a, c, lo, hi);
+    return mont_reduce(lo, hi);
+  }
+  // Host: plain canonical value -> Montgomery constant form (times 2^64 = EPS mod P).
+  HB_HD static u64 to_mont(u64 v) { return canon(mul(v, EPS)); }
 };
 
 // -------------------------------- BabyBear --------------------------------

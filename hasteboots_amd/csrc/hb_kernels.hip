@@ -87,7 +87,8 @@ __device__ __forceinline__ void load4(const u32* __restrict__ p, u32 (&v)[4]) {
   v[3] = x.w;
 }
 
-// acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each.
+// acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each. Goldilocks
+// accumulators stay in [0, 2^64) (the inverse transform canonicalises its input).
 // ntt_add_mul_assign_fast (algebra/src/polynomial/univariate/ntt_polynomial.rs:620-629).
 template <class F, int E>
 __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
@@ -103,8 +104,12 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
     load4(kb + slot, vb);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      pa[4 * q + r] = DF<F>::add(pa[4 * q + r], DF<F>::mul(x[4 * q + r], va[r]));
-      pb[4 * q + r] = DF<F>::add(pb[4 * q + r], DF<F>::mul(x[4 * q + r], vb[r]));
+      if constexpr (F::IS_GOLDILOCKS) {
+        gl_mac2(pa[4 * q + r], pb[4 * q + r], x[4 * q + r], va[r], vb[r]);
+      } else {
+        pa[4 * q + r] = DF<F>::add(pa[4 * q + r], DF<F>::mul(x[4 * q + r], va[r]));
+        pb[4 * q + r] = DF<F>::add(pb[4 * q + r], DF<F>::mul(x[4 * q + r], vb[r]));
+      }
     }
   }
 }

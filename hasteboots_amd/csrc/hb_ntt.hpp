@@ -41,7 +41,8 @@ struct DF<Gl> {
   using W = u64;
   __device__ __forceinline__ static W add(W a, W b) { return Gl::add(a, b); }
   __device__ __forceinline__ static W sub(W a, W b) { return Gl::sub(a, b); }
-  __device__ __forceinline__ static W mul(W a, W c) { return Gl::mul(a, c); }
+  // Constants (twiddles, keys) are stored times 2^64 (Montgomery constant form).
+  __device__ __forceinline__ static W mul(W a, W c) { return Gl::mont_mul(a, c); }
   __device__ __forceinline__ static W canon(W a) { return Gl::canon(a); }
 };
 
@@ -71,6 +72,82 @@ __device__ __forceinline__ void gs_bfly(typename DF<F>::W& x, typename DF<F>::W&
   x = s;
 }
 
+// Pairs of independent butterflies. The generic form is two scalar butterflies;
+// Goldilocks uses the hand-scheduled gfx950 sequences of hb_gl_asm.inc, which
+// interleave the two carry chains (t = y*w is a Montgomery product, canonical; CT
+// outputs lie in [0, 2^64); GS takes and returns canonical words). `_u` variants take
+// wave-uniform twiddles (SGPRs), `_v` per-lane twiddles.
+template <class F>
+struct Bfly {
+  using W = typename DF<F>::W;
+  __device__ __forceinline__ static void ct2_u(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    ct_bfly<F>(xa, ya, wa);
+    ct_bfly<F>(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void ct2_v(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    ct_bfly<F>(xa, ya, wa);
+    ct_bfly<F>(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void gs2_u(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    gs_bfly<F>(xa, ya, wa);
+    gs_bfly<F>(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void gs2_v(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    gs_bfly<F>(xa, ya, wa);
+    gs_bfly<F>(xb, yb, wb);
+  }
+  // Inverse-transform input normalisation (identity: inputs are canonical).
+  __device__ __forceinline__ static W inv_in(W x) { return x; }
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#include "hb_gl_asm.inc"
+
+template <>
+struct Bfly<Gl> {
+  using W = u64;
+  __device__ __forceinline__ static void ct2_u(u64& xa, u64& ya, u64 wa, u64& xb, u64& yb, u64 wb) {
+    u64 Xa, Ya, Xb, Yb, dummy, s1a, s2a, s1b, s2b;
+    HB_GL_CT2_S_ASM;
+    xa = Xa, ya = Ya, xb = Xb, yb = Yb;
+  }
+  __device__ __forceinline__ static void ct2_v(u64& xa, u64& ya, u64 wa, u64& xb, u64& yb, u64 wb) {
+    u64 Xa, Ya, Xb, Yb, dummy, s1a, s2a, s1b, s2b;
+    HB_GL_CT2_V_ASM;
+    xa = Xa, ya = Ya, xb = Xb, yb = Yb;
+  }
+  __device__ __forceinline__ static void gs2_u(u64& xa, u64& ya, u64 wa, u64& xb, u64& yb, u64 wb) {
+    u32 Xalo, Xahi, Xblo, Xbhi;
+    u64 Ya, Yb, dummy, s1a, s2a, s1b, s2b;
+    HB_GL_GS2_S_ASM;
+    xa = ((u64)Xahi << 32) | Xalo, ya = Ya, xb = ((u64)Xbhi << 32) | Xblo, yb = Yb;
+  }
+  __device__ __forceinline__ static void gs2_v(u64& xa, u64& ya, u64 wa, u64& xb, u64& yb, u64 wb) {
+    u32 Xalo, Xahi, Xblo, Xbhi;
+    u64 Ya, Yb, dummy, s1a, s2a, s1b, s2b;
+    HB_GL_GS2_V_ASM;
+    xa = ((u64)Xahi << 32) | Xalo, ya = Ya, xb = ((u64)Xbhi << 32) | Xblo, yb = Yb;
+  }
+  __device__ __forceinline__ static u64 inv_in(u64 x) { return Gl::canon(x); }
+};
+
+// (pa, pb) += (x * ka, x * kb) with Montgomery-form keys; pa, pb in [0, 2^64).
+__device__ __forceinline__ void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb) {
+  u64 Pa, Pb, dummy, s1a, s2a, s1b, s2b;
+  HB_GL_MAC2_ASM;
+  pa = Pa, pb = Pb;
+}
+#else
+// Host compilation pass only parses device code; this body is never compiled for a GPU.
+__device__ inline void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb) {
+  pa = Gl::add(pa, Gl::mont_mul(x, ka));
+  pb = Gl::add(pb, Gl::mont_mul(x, kb));
+}
+#endif
+
+// Index of the k-th butterfly's upper element in a stage whose partner distance is half.
+__host__ __device__ constexpr int bf_lo(int k, int half) { return (k / half) * 2 * half + k % half; }
+
 template <int E>
 struct Log2 {
   static constexpr int value = E == 16 ? 4 : E == 32 ? 5 : E == 8 ? 3 : -1;
@@ -90,10 +167,10 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
     const int m = 1 << s;
     const int half = (E / 2) >> s;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      if (e & half) continue;
-      const W w = tw[m + e / (2 * half)];
-      ct_bfly<F>(x[e], x[e + half], w);
+    for (int p = 0; p < E / 4; p++) {
+      const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
+      Bfly<F>::ct2_u(x[e0], x[e0 + half], tw[m + e0 / (2 * half)], x[e1], x[e1 + half],
+                     tw[m + e1 / (2 * half)]);
     }
   }
 #pragma unroll
@@ -116,10 +193,10 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
       const int m = E << s;
       const int half = 8 >> s;
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        if (k & half) continue;
-        const W w = tw[m + (b << s) + k / (2 * half)];
-        ct_bfly<F>(x[16 * v + k], x[16 * v + k + half], w);
+      for (int p = 0; p < 4; p++) {
+        const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
+        Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], tw[m + (b << s) + k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], tw[m + (b << s) + k1 / (2 * half)]);
       }
     }
   }
@@ -143,12 +220,10 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
   for (int q = 0; q < E / 4; q++) {
     const int c = lane + 64 * q;
     const W w0 = tw[16 * E + c];
-    ct_bfly<F>(x[4 * q + 0], x[4 * q + 2], w0);
-    ct_bfly<F>(x[4 * q + 1], x[4 * q + 3], w0);
+    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
     const W w1 = tw[32 * E + 2 * c];
     const W w2 = tw[32 * E + 2 * c + 1];
-    ct_bfly<F>(x[4 * q + 0], x[4 * q + 1], w1);
-    ct_bfly<F>(x[4 * q + 2], x[4 * q + 3], w2);
+    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
   }
 }
 
@@ -159,17 +234,17 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
                                              const typename DF<F>::W* __restrict__ itw, int lane) {
   using W = typename DF<F>::W;
   constexpr int LOGE = Log2<E>::value;
+#pragma unroll
+  for (int e = 0; e < E; e++) x[e] = Bfly<F>::inv_in(x[e]);
   // pass 3
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
     const int c = lane + 64 * q;
     const W w1 = itw[32 * E + 2 * c];
     const W w2 = itw[32 * E + 2 * c + 1];
-    gs_bfly<F>(x[4 * q + 0], x[4 * q + 1], w1);
-    gs_bfly<F>(x[4 * q + 2], x[4 * q + 3], w2);
+    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
     const W w0 = itw[16 * E + c];
-    gs_bfly<F>(x[4 * q + 0], x[4 * q + 2], w0);
-    gs_bfly<F>(x[4 * q + 1], x[4 * q + 3], w0);
+    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
   }
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
@@ -195,10 +270,10 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
       const int m = E << s;
       const int half = 8 >> s;
 #pragma unroll
-      for (int k = 0; k < 16; k++) {
-        if (k & half) continue;
-        const W w = itw[m + (b << s) + k / (2 * half)];
-        gs_bfly<F>(x[16 * v + k], x[16 * v + k + half], w);
+      for (int p = 0; p < 4; p++) {
+        const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
+        Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], itw[m + (b << s) + k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], itw[m + (b << s) + k1 / (2 * half)]);
       }
     }
   }
@@ -219,14 +294,16 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
     const int m = 1 << s;
     const int half = (E / 2) >> s;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      if (e & half) continue;
-      const W w = itw[m + e / (2 * half)];
-      gs_bfly<F>(x[e], x[e + half], w);
+    for (int p = 0; p < E / 4; p++) {
+      const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
+      Bfly<F>::gs2_u(x[e0], x[e0 + half], itw[m + e0 / (2 * half)], x[e1], x[e1 + half],
+                     itw[m + e1 / (2 * half)]);
     }
   }
+  if constexpr (!F::IS_GOLDILOCKS) {  // the Goldilocks GS pair already returns canonical words
 #pragma unroll
-  for (int e = 0; e < E; e++) x[e] = DF<F>::canon(x[e]);
+    for (int e = 0; e < E; e++) x[e] = DF<F>::canon(x[e]);
+  }
 }
 
 }  // namespace hb

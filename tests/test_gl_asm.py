@@ -1,0 +1,190 @@
+"""CPU checks of the hand-scheduled Goldilocks sequences (hasteboots_amd/csrc/hb_gl_asm.inc).
+
+The generated file must match its generator, and every macro is run through a one-lane
+interpreter of the ten gfx950 instructions it uses, against Python big-integer
+arithmetic: CT and GS butterflies and the MAC with Montgomery-form constants
+(c stands for c * 2^-64 mod P), on random and edge words.
+"""
+import random
+import re
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+CSRC = ROOT / "hasteboots_amd" / "csrc"
+INC = CSRC / "hb_gl_asm.inc"
+P = 2**64 - 2**32 + 1
+EPS = 2**32 - 1
+M32, M64 = 2**32 - 1, 2**64 - 1
+RINV = pow(2**64, P - 2, P)
+EDGE = [0, 1, 2, EPS, 2**32, 2**63, P - 2, P - 1, P, P + 1, M64 - 1, M64, 2**64 - 2**32]
+
+
+def parse_macros():
+    text = INC.read_text()
+    macros = {}
+    for m in re.finditer(r"#define (HB_GL_\w+_ASM) \\\n  asm\( \\\n(.*?)\)\n", text, re.S):
+        body = m.group(2)
+        lines = re.findall(r'^\s+"(.*?)\\n" \\$', body, re.M)
+        outs, ins = re.findall(r"^\s+: (.*?) \\$", body, re.M)[:2]
+        ops = re.findall(r'"([=&a-z]+)"\(([^"]*?)\)(?:, |$)', outs + ", " + ins + ", ")
+        macros[m.group(1)] = (lines, ops)
+    return macros
+
+
+def c_eval(expr, env):
+    e = expr.replace("0x55555555u", "0x55555555")
+    e = re.sub(r"\(u32\)\(([^()]*)\)", r"u32(\1)", e)
+    e = re.sub(r"\(u32\)(\w+)", r"u32(\1)", e)
+    return eval(e, {"u32": lambda v: v & M32}, env)
+
+
+class Lane:
+    """One lane of the VALU: 32-bit VGPRs, SGPR pairs reduced to this lane's bit."""
+
+    def __init__(self, operands):
+        self.v = {}
+        self.op = dict(operands)  # operand index -> value (64-bit for pairs)
+
+    def rd32(self, a):
+        if a.startswith("%"):
+            return self.op[int(a[1:])] & M32
+        if re.fullmatch(r"v\d+", a):
+            return self.v[int(a[1:])]
+        return int(a) & M32
+
+    def rd64(self, a):
+        if a.startswith("%"):
+            return self.op[int(a[1:])] & M64
+        m = re.fullmatch(r"v\[(\d+):(\d+)\]", a)
+        if m:
+            lo = int(m.group(1))
+            return self.v[lo] | (self.v[lo + 1] << 32)
+        return int(a) & M64
+
+    def wr32(self, a, x):
+        if a.startswith("%"):
+            self.op[int(a[1:])] = x & M32
+        else:
+            self.v[int(a[1:])] = x & M32
+
+    def wr64(self, a, x):
+        if a.startswith("%"):
+            self.op[int(a[1:])] = x & M64
+        else:
+            lo = int(re.fullmatch(r"v\[(\d+):\d+\]", a).group(1))
+            self.v[lo], self.v[lo + 1] = x & M32, (x >> 32) & M32
+
+    def bit(self, a):
+        return self.op[int(a[1:])] & 1
+
+    def run(self, lines):
+        for ln in lines:
+            mn, _, rest = ln.partition(" ")
+            a = [x.strip() for x in rest.split(",")] if rest else []
+            if mn == "s_nop":
+                continue
+            if mn == "v_mov_b32":
+                self.wr32(a[0], self.rd32(a[1]))
+            elif mn == "v_mov_b64":
+                self.wr64(a[0], self.rd64(a[1]))
+            elif mn == "v_mad_u64_u32":
+                r = self.rd32(a[2]) * self.rd32(a[3]) + self.rd64(a[4])
+                self.wr64(a[0], r)
+                self.op[int(a[1][1:])] = r >> 64
+            elif mn == "v_mad_i64_i32":
+                def s32(x):
+                    return x - 2**32 if x >> 31 else x
+                c = self.rd64(a[4])
+                c = c - 2**64 if c >> 63 else c
+                self.wr64(a[0], s32(self.rd32(a[2])) * s32(self.rd32(a[3])) + c)
+            elif mn in ("v_add_co_u32", "v_addc_co_u32"):
+                assert len(a) == (5 if mn == "v_addc_co_u32" else 4), ln
+                r = self.rd32(a[2]) + self.rd32(a[3]) + (self.bit(a[4]) if len(a) > 4 else 0)
+                self.wr32(a[0], r)
+                self.op[int(a[1][1:])] = r >> 32
+            elif mn in ("v_sub_co_u32", "v_subb_co_u32"):
+                assert len(a) == (5 if mn == "v_subb_co_u32" else 4), ln
+                r = self.rd32(a[2]) - self.rd32(a[3]) - (self.bit(a[4]) if len(a) > 4 else 0)
+                self.wr32(a[0], r)
+                self.op[int(a[1][1:])] = 1 if r < 0 else 0
+            elif mn == "v_cndmask_b32":
+                self.wr32(a[0], self.rd32(a[2]) if self.bit(a[3]) else self.rd32(a[1]))
+            elif mn == "s_or_b64":
+                self.op[int(a[0][1:])] = self.bit(a[1]) | self.bit(a[2])
+            else:
+                raise AssertionError(f"unknown instruction {ln}")
+
+
+def run_macro(macro, env):
+    lines, ops = macro
+    vals = {}
+    for i, (con, expr) in enumerate(ops):
+        if "=" not in con:
+            vals[i] = c_eval(expr, env)
+    lane = Lane(vals)
+    lane.run(lines)
+    return {expr: lane.op[i] for i, (con, expr) in enumerate(ops) if con in ("=v",)}
+
+
+def mont(y, c):
+    return y * c * RINV % P
+
+
+def samples(rng, n, canonical=False):
+    top = P if canonical else 2**64
+    out = [rng.randrange(top) for _ in range(n)]
+    out += [e for e in EDGE if e < top]
+    return out
+
+
+def test_generated_file_is_current():
+    r = subprocess.run([sys.executable, str(CSRC / "gen_gl_asm.py"), "--check"])
+    assert r.returncode == 0, "hb_gl_asm.inc differs from gen_gl_asm.py output"
+
+
+@pytest.mark.parametrize("name", ["HB_GL_CT2_V_ASM", "HB_GL_CT2_S_ASM"])
+def test_ct_pair(name):
+    mac = parse_macros()[name]
+    rng = random.Random(1)
+    xs, ys, ws = samples(rng, 300), samples(rng, 300), samples(rng, 300, canonical=True)
+    for i in range(len(xs)):
+        env = dict(xa=xs[i], ya=ys[i], wa=ws[i % len(ws)], xb=ys[-1 - i], yb=xs[i],
+                   wb=ws[(7 * i) % len(ws)])
+        out = run_macro(mac, env)
+        for t in "ab":
+            x, y, w = env["x" + t], env["y" + t], env["w" + t]
+            tt = mont(y, w)
+            assert out["X" + t] % P == (x + tt) % P
+            assert out["Y" + t] % P == (x - tt) % P
+
+
+@pytest.mark.parametrize("name", ["HB_GL_GS2_V_ASM", "HB_GL_GS2_S_ASM"])
+def test_gs_pair(name):
+    mac = parse_macros()[name]
+    rng = random.Random(2)
+    xs, ys, ws = (samples(rng, 300, canonical=True) for _ in range(3))
+    for i in range(len(xs)):
+        env = dict(xa=xs[i], ya=ys[i], wa=ws[i], xb=ys[-1 - i], yb=xs[-1 - i],
+                   wb=ws[(5 * i) % len(ws)])
+        out = run_macro(mac, env)
+        for t in "ab":
+            x, y, w = env["x" + t], env["y" + t], env["w" + t]
+            sx = out[f"X{t}lo"] | (out[f"X{t}hi"] << 32)
+            assert sx == (x + y) % P  # canonical sum
+            assert out["Y" + t] == mont((x - y) % P, w)  # canonical product
+
+
+def test_mac_pair():
+    mac = parse_macros()["HB_GL_MAC2_ASM"]
+    rng = random.Random(3)
+    xs, ps = samples(rng, 300), samples(rng, 300)
+    ks = samples(rng, 300, canonical=True)
+    for i in range(len(xs)):
+        env = dict(x=xs[i], ka=ks[i % len(ks)], kb=ks[-1 - i % len(ks)], pa=ps[i], pb=ps[-1 - i])
+        out = run_macro(mac, env)
+        assert out["Pa"] % P == (env["pa"] + mont(env["x"], env["ka"])) % P
+        assert out["Pb"] % P == (env["pb"] + mont(env["x"], env["kb"])) % P
