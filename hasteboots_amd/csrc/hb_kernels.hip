@@ -90,18 +90,30 @@ __device__ __forceinline__ void load4(const u32* __restrict__ p, u32 (&v)[4]) {
 // acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each. Goldilocks
 // accumulators stay in [0, 2^64) (the inverse transform canonicalises its input).
 // ntt_add_mul_assign_fast (algebra/src/polynomial/univariate/ntt_polynomial.rs:620-629).
+// Key words are software-pipelined one 4-slot group ahead: group 0 arrives in
+// (ka0, kb0), loaded by the caller before the forward transform, and group q+1 is in
+// flight while group q is multiplied. The scheduling barriers keep each load ahead of
+// the arithmetic it overlaps.
 template <class F, int E>
 __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
                                         const typename DF<F>::W (&x)[E],
                                         const typename DF<F>::W* __restrict__ ka,
-                                        const typename DF<F>::W* __restrict__ kb, int lane) {
+                                        const typename DF<F>::W* __restrict__ kb,
+                                        const typename DF<F>::W (&ka0)[4],
+                                        const typename DF<F>::W (&kb0)[4], int lane) {
   using W = typename DF<F>::W;
+  W va[4], vb[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) va[r] = ka0[r], vb[r] = kb0[r];
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
-    const int slot = 4 * (lane + 64 * q);
-    W va[4], vb[4];
-    load4(ka + slot, va);
-    load4(kb + slot, vb);
+    W na[4], nb[4];
+    if (q + 1 < E / 4) {
+      const int slot = 4 * (lane + 64 * (q + 1));
+      load4(ka + slot, na);
+      load4(kb + slot, nb);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       if constexpr (F::IS_GOLDILOCKS) {
@@ -111,29 +123,40 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
         pb[4 * q + r] = DF<F>::add(pb[4 * q + r], DF<F>::mul(x[4 * q + r], vb[r]));
       }
     }
+    __builtin_amdgcn_sched_barrier(0);
+    if (q + 1 < E / 4) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) va[r] = na[r], vb[r] = nb[r];
+    }
   }
 }
 
 // Σ over one gadget of digit NTTs: for j < len, x = NTT(digit_j(src)), acc += x ⊙ key_j.
 // Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
-// values, digit_j = (v >> j*bits) & mask.
+// values, digit_j = (v >> j*bits) & mask. tw: forward twiddles in global memory (pass 1),
+// twv: the same table for per-lane reads (LDS copy).
 template <class F, int E>
 __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
                                            const typename DF<F>::W (&src)[E], u32 len, u32 bits,
                                            const typename DF<F>::W* __restrict__ key,
                                            typename DF<F>::W* lds,
-                                           const typename DF<F>::W* __restrict__ tw, int lane) {
+                                           const typename DF<F>::W* __restrict__ tw,
+                                           const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
   constexpr int N = 64 * E;
   const W mask = (W)((1u << bits) - 1u);
   for (u32 j = 0; j < len; j++) {
+    const W* ka = key + (size_t)(2 * j) * N;
+    W ka0[4], kb0[4];
+    load4(ka + 4 * lane, ka0);
+    load4(ka + N + 4 * lane, kb0);
+    __builtin_amdgcn_sched_barrier(0);
     W x[E];
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
-    ntt_fwd_wave<F, E>(x, lds, tw, lane);
-    const W* ka = key + (size_t)(2 * j) * N;
-    mac_row<F, E>(pa, pb, x, ka, ka + N, lane);
+    ntt_fwd_wave<F, E>(x, lds, tw, twv, lane);
+    mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
   }
 }
 
@@ -157,7 +180,9 @@ __device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
 // ----------------------------------------------------------------------------
 template <class F>
 __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
-                                               unsigned short* ms) {
+                                               unsigned short* ms,
+                                               const typename DF<F>::W* twl,
+                                               const typename DF<F>::W* itwl) {
   using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
@@ -269,17 +294,17 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
       }
       // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
       gadget_mac<F, E>(pa, pb, dst, A.br_len, A.br_bits,
-                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, lane);
+                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, twl, lane);
     }
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
     // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
-    ntt_inv_wave<F, E>(pa, lds, itw, lane);
+    ntt_inv_wave<F, E>(pa, lds, itw, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
       outA[k] = F::add(accA[k], pa[e]);
     }
-    ntt_inv_wave<F, E>(pb, lds, itw, lane);
+    ntt_inv_wave<F, E>(pb, lds, itw, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
@@ -299,14 +324,14 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   W pa[E], pb[E];
 #pragma unroll
   for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
-  gadget_mac<F, E>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, lane);
+  gadget_mac<F, E>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, twl, lane);
   // b' = coefficient 0 of INTT(init.b + Σ) = b0 + Σ_slots Σ (1/N folded into the KSK).
   W s = 0;
 #pragma unroll
   for (int e = 0; e < E; e++) s = DF<F>::add(s, pb[e]);
   s = wave_sum<F>(s);
   const W bout = F::canon(DF<F>::add(F::canon(b0), s));
-  ntt_inv_wave<F, E>(pa, lds, itw, lane);
+  ntt_inv_wave<F, E>(pa, lds, itw, itwl, lane);
   // extract_partial_lwe_locally (rlwe.rs:393-402): a' = [A0, -A_{N-1}, ..., -A_1][..n].
   u64* out = A.out + (size_t)(wi ? wi[2] : g) * n1;
 #pragma unroll
@@ -322,12 +347,28 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   if (lane == 0) out[n] = (u64)bout;
 }
 
+// Copies the forward and inverse twiddle tables (N words each) into LDS for the
+// per-lane reads of NTT passes 2 and 3.
+template <class W>
+__device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, W* twl, W* itwl,
+                                               int n, int tid, int nthreads) {
+  const W* a = static_cast<const W*>(tw);
+  const W* b = static_cast<const W*>(itw);
+  for (int k = tid; k < n; k += nthreads) {
+    twl[k] = a[k];
+    itwl[k] = b[k];
+  }
+  __syncthreads();
+}
+
 template <class F>
 __global__ __launch_bounds__(64) void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[1024 + 64];
   __shared__ unsigned short ms[1024];
-  bootstrap_gate<F>(A, blockIdx.x, lds, ms);
+  __shared__ W twl[2][1024];
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64);
+  bootstrap_gate<F>(A, blockIdx.x, lds, ms, twl[0], twl[1]);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -336,15 +377,20 @@ __global__ __launch_bounds__(64) void bootstrap_kernel(BootArgs A) {
 __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[1024 + 64];
   __shared__ unsigned short ms[1024];
+  __shared__ u64 twl[2][1024];
   const u32 b = blockIdx.x;
   int p = 0;
   while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
   const BootArgs A = parts[p];
   const int g = (int)(b - A.first_block);
-  if (A.is_goldilocks)
-    bootstrap_gate<Gl>(A, g, lds, ms);
-  else
-    bootstrap_gate<Bb>(A, g, reinterpret_cast<u32*>(lds), ms);
+  if (A.is_goldilocks) {
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64);
+    bootstrap_gate<Gl>(A, g, lds, ms, twl[0], twl[1]);
+  } else {
+    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64);
+    bootstrap_gate<Bb>(A, g, reinterpret_cast<u32*>(lds), ms, t, t + 1024);
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -358,17 +404,20 @@ __global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t
   using W = typename DF<F>::W;
   constexpr int N = 64 * E;
   __shared__ W lds_all[4][N + N / 16];
+  __shared__ W twl[N];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const W* tw = static_cast<const W*>(tw_);
+  for (int k = threadIdx.x; k < N; k += 256) twl[k] = tw[k];
+  __syncthreads();
   const size_t poly = (size_t)blockIdx.x * 4 + wave;
   if (poly >= count) return;
-  const W* tw = static_cast<const W*>(tw_);
   W* lds = lds_all[wave];
   u64* d = data + poly * N;
   W x[E];
   if (!inverse) {
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (W)d[lane + 64 * e];
-    ntt_fwd_wave<F, E>(x, lds, tw, lane);
+    ntt_fwd_wave<F, E>(x, lds, tw, twl, lane);
 #pragma unroll
     for (int q = 0; q < E / 4; q++) {
       const int p = 4 * (lane + 64 * q);
@@ -387,7 +436,7 @@ __global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t
       x[4 * q + 2] = (W)b.x;
       x[4 * q + 3] = (W)b.y;
     }
-    ntt_inv_wave<F, E>(x, lds, tw, lane);
+    ntt_inv_wave<F, E>(x, lds, tw, twl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) d[lane + 64 * e] = (u64)DF<F>::canon(DF<F>::mul(x[e], ninv));
   }
@@ -404,7 +453,9 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
   using W = typename DF<F>::W;
   constexpr int E = 16, N = 1024;
   __shared__ W lds_all[4][N + N / 16];
+  __shared__ W twl[2][N];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  stage_twiddles<W>(tw_, itw_, twl[0], twl[1], N, threadIdx.x, 256);
   const size_t idx = (size_t)blockIdx.x * 4 + wave;
   if (idx >= count) return;
   const W* rgsw = static_cast<const W*>(rgsw_);
@@ -420,10 +471,11 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
     W src[E];
 #pragma unroll
     for (int e = 0; e < E; e++) src[e] = (W)d[comp * N + lane + 64 * e];
-    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, tw, lane);
+    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, tw, twl[0],
+                     lane);
   }
-  ntt_inv_wave<F, E>(pa, lds, itw, lane);
-  ntt_inv_wave<F, E>(pb, lds, itw, lane);
+  ntt_inv_wave<F, E>(pa, lds, itw, twl[1], lane);
+  ntt_inv_wave<F, E>(pb, lds, itw, twl[1], lane);
 #pragma unroll
   for (int e = 0; e < E; e++) {
     d[lane + 64 * e] = (u64)pa[e];

@@ -154,10 +154,13 @@ struct Log2 {
 };
 
 // Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
-// tw = psi_rev (psi^bitrev(k), constants in the field's constant form).
+// tw = psi_rev (psi^bitrev(k), constants in the field's constant form) in global memory:
+// pass 1 reads it at wave-uniform indices (scalar loads). twv is the same table for the
+// per-lane reads of passes 2 and 3, normally a copy in LDS.
 template <class F, int E>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
-                                             const typename DF<F>::W* __restrict__ tw, int lane) {
+                                             const typename DF<F>::W* __restrict__ tw,
+                                             const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
   constexpr int LOGE = Log2<E>::value;
   static_assert(LOGE > 0, "E must be 8, 16 or 32");
@@ -195,8 +198,8 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
-        Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], tw[m + (b << s) + k0 / (2 * half)],
-                       x[16 * v + k1], x[16 * v + k1 + half], tw[m + (b << s) + k1 / (2 * half)]);
+        Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], twv[m + (b << s) + k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], twv[m + (b << s) + k1 / (2 * half)]);
       }
     }
   }
@@ -219,10 +222,10 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
     const int c = lane + 64 * q;
-    const W w0 = tw[16 * E + c];
+    const W w0 = twv[16 * E + c];
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
-    const W w1 = tw[32 * E + 2 * c];
-    const W w2 = tw[32 * E + 2 * c + 1];
+    const W w1 = twv[32 * E + 2 * c];
+    const W w2 = twv[32 * E + 2 * c + 1];
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
   }
 }
@@ -231,7 +234,8 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 // Out: x[e] = position lane + 64e, canonical. itw = psi^-bitrev(k).
 template <class F, int E>
 __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
-                                             const typename DF<F>::W* __restrict__ itw, int lane) {
+                                             const typename DF<F>::W* __restrict__ itw,
+                                             const typename DF<F>::W* itwv, int lane) {
   using W = typename DF<F>::W;
   constexpr int LOGE = Log2<E>::value;
 #pragma unroll
@@ -240,10 +244,10 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
     const int c = lane + 64 * q;
-    const W w1 = itw[32 * E + 2 * c];
-    const W w2 = itw[32 * E + 2 * c + 1];
+    const W w1 = itwv[32 * E + 2 * c];
+    const W w2 = itwv[32 * E + 2 * c + 1];
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
-    const W w0 = itw[16 * E + c];
+    const W w0 = itwv[16 * E + c];
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
   }
 #pragma unroll
@@ -272,8 +276,8 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
-        Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], itw[m + (b << s) + k0 / (2 * half)],
-                       x[16 * v + k1], x[16 * v + k1 + half], itw[m + (b << s) + k1 / (2 * half)]);
+        Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], itwv[m + (b << s) + k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], itwv[m + (b << s) + k1 / (2 * half)]);
       }
     }
   }
