@@ -7,7 +7,7 @@
 // split into three register passes joined by two exchanges through a padded LDS
 // buffer of N + N/16 words:
 //   pass 1: log2(E) stages; lane l holds positions l + 64e (twiddles are uniform
-//           across the wave, so they come from the scalar cache);
+//           across the wave and compile-time constants);
 //   pass 2: 4 stages inside each 64-word block; lane holds positions
 //           64b + j0 + 4k of block b = u>>2, j0 = u&3 for u = l + 64v;
 //   pass 3: 2 stages inside each 4-word block; lane holds positions 4(l+64q)+r,
@@ -145,6 +145,42 @@ __device__ inline void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb) {
 }
 #endif
 
+// Pass-1 twiddles as compile-time constants. psi = g^((P-1)/2N) (g = 7 Goldilocks,
+// 31 BabyBear) exactly as the host tables; entry k < E is psi^bitrev(k) (or psi^-bitrev(k))
+// in the field's constant form (times 2^64 Goldilocks, 2^32 BabyBear). Pass 1 only
+// touches indices below E, so these wave-uniform constants replace table loads there.
+__host__ __device__ constexpr u64 ct_mulmod(u64 a, u64 b, u64 p) {
+  return (u64)((unsigned __int128)a * b % p);
+}
+__host__ __device__ constexpr u64 ct_powmod(u64 b, u64 e, u64 p) {
+  u64 r = 1;
+  b %= p;
+  while (e) {
+    if (e & 1) r = ct_mulmod(r, b, p);
+    b = ct_mulmod(b, b, p);
+    e >>= 1;
+  }
+  return r;
+}
+__host__ __device__ constexpr u64 ct_bitrev(u64 x, int bits) {
+  u64 r = 0;
+  for (int i = 0; i < bits; i++) r |= ((x >> i) & 1) << (bits - 1 - i);
+  return r;
+}
+template <class F, int LOGN, bool INV>
+struct Pass1Tw {
+  using W = typename DF<F>::W;
+  static constexpr int E = (1 << LOGN) / 64;
+  W w[E];
+  __host__ __device__ constexpr Pass1Tw() : w{} {
+    const u64 p = F::P;
+    const u64 psi = ct_powmod(F::GENERATOR, (p - 1) >> (LOGN + 1), p);
+    const u64 base = INV ? ct_powmod(psi, p - 2, p) : psi;
+    const u64 r = F::IS_GOLDILOCKS ? (u64)(((unsigned __int128)1 << 64) % p) : ((u64)1 << 32) % p;
+    for (int k = 0; k < E; k++)
+      w[k] = (W)ct_mulmod(ct_powmod(base, ct_bitrev(k, LOGN), p), r, p);
+  }
+};
 // Index of the k-th butterfly's upper element in a stage whose partner distance is half.
 __host__ __device__ constexpr int bf_lo(int k, int half) { return (k / half) * 2 * half + k % half; }
 
@@ -154,8 +190,8 @@ struct Log2 {
 };
 
 // Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
-// tw = psi_rev (psi^bitrev(k), constants in the field's constant form) in global memory:
-// pass 1 reads it at wave-uniform indices (scalar loads). twv is the same table for the
+// tw = psi_rev (psi^bitrev(k), constants in the field's constant form); pass 1 uses the
+// compile-time copy of its first E entries (Pass1Tw). twv is the same table for the
 // per-lane reads of passes 2 and 3, normally a copy in LDS.
 template <class F, int E>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
@@ -172,8 +208,9 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
     for (int p = 0; p < E / 4; p++) {
       const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
-      Bfly<F>::ct2_u(x[e0], x[e0 + half], tw[m + e0 / (2 * half)], x[e1], x[e1 + half],
-                     tw[m + e1 / (2 * half)]);
+      constexpr Pass1Tw<F, LOGE + 6, false> T{};
+      Bfly<F>::ct2_u(x[e0], x[e0 + half], T.w[m + e0 / (2 * half)], x[e1], x[e1 + half],
+                     T.w[m + e1 / (2 * half)]);
     }
   }
 #pragma unroll
@@ -300,8 +337,9 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
     for (int p = 0; p < E / 4; p++) {
       const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
-      Bfly<F>::gs2_u(x[e0], x[e0 + half], itw[m + e0 / (2 * half)], x[e1], x[e1 + half],
-                     itw[m + e1 / (2 * half)]);
+      constexpr Pass1Tw<F, LOGE + 6, true> T{};
+      Bfly<F>::gs2_u(x[e0], x[e0 + half], T.w[m + e0 / (2 * half)], x[e1], x[e1 + half],
+                     T.w[m + e1 / (2 * half)]);
     }
   }
   if constexpr (!F::IS_GOLDILOCKS) {  // the Goldilocks GS pair already returns canonical words
