@@ -155,7 +155,7 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
-    ntt_fwd_wave<F, E>(x, lds, tw, twv, lane);
+    ntt_fwd_wave<F, E>(x, lds, twv, lane);
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
   }
 }
@@ -298,13 +298,13 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
     }
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
     // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
-    ntt_inv_wave<F, E>(pa, lds, itw, itwl, lane);
+    ntt_inv_wave<F, E>(pa, lds, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
       outA[k] = F::add(accA[k], pa[e]);
     }
-    ntt_inv_wave<F, E>(pb, lds, itw, itwl, lane);
+    ntt_inv_wave<F, E>(pb, lds, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
@@ -331,7 +331,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   for (int e = 0; e < E; e++) s = DF<F>::add(s, pb[e]);
   s = wave_sum<F>(s);
   const W bout = F::canon(DF<F>::add(F::canon(b0), s));
-  ntt_inv_wave<F, E>(pa, lds, itw, itwl, lane);
+  ntt_inv_wave<F, E>(pa, lds, itwl, lane);
   // extract_partial_lwe_locally (rlwe.rs:393-402): a' = [A0, -A_{N-1}, ..., -A_1][..n].
   u64* out = A.out + (size_t)(wi ? wi[2] : g) * n1;
 #pragma unroll
@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t
   if (!inverse) {
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (W)d[lane + 64 * e];
-    ntt_fwd_wave<F, E>(x, lds, tw, twl, lane);
+    ntt_fwd_wave<F, E>(x, lds, twl, lane);
 #pragma unroll
     for (int q = 0; q < E / 4; q++) {
       const int p = 4 * (lane + 64 * q);
@@ -436,7 +436,7 @@ __global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t
       x[4 * q + 2] = (W)b.x;
       x[4 * q + 3] = (W)b.y;
     }
-    ntt_inv_wave<F, E>(x, lds, tw, twl, lane);
+    ntt_inv_wave<F, E>(x, lds, twl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) d[lane + 64 * e] = (u64)DF<F>::canon(DF<F>::mul(x[e], ninv));
   }
@@ -474,8 +474,8 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
     gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, tw, twl[0],
                      lane);
   }
-  ntt_inv_wave<F, E>(pa, lds, itw, twl[1], lane);
-  ntt_inv_wave<F, E>(pb, lds, itw, twl[1], lane);
+  ntt_inv_wave<F, E>(pa, lds, twl[1], lane);
+  ntt_inv_wave<F, E>(pb, lds, twl[1], lane);
 #pragma unroll
   for (int e = 0; e < E; e++) {
     d[lane + 64 * e] = (u64)pa[e];

@@ -31,8 +31,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int lds_pad(int p) { return p + ((p >> 6) << 2); }
-
 template <class F>
 struct DF;
 
@@ -193,13 +191,31 @@ struct Log2 {
 // tw = psi_rev (psi^bitrev(k), constants in the field's constant form); pass 1 uses the
 // compile-time copy of its first E entries (Pass1Tw). twv is the same table for the
 // per-lane reads of passes 2 and 3, normally a copy in LDS.
+// LDS word offsets of the three register layouts (buffer of N + N/16 words, one pad of 4
+// words per 64). With lane < 64 each padded index is a per-lane base plus a constant,
+// so every exchange access is one base register and an immediate offset:
+//   layout 1 (lane + 64e):               lane + 68e
+//   layout 2 (64(u>>2) + (u&3) + 4k,      68(lane>>2) + (lane&3) + 1088v + 4k
+//             u = lane + 64v)
+//   layout 3 (4(lane + 64q) + r):         4lane + 4(lane>>4) + 272q + r
+__device__ __forceinline__ int lds_base2(int lane) { return 68 * (lane >> 2) + (lane & 3); }
+__device__ __forceinline__ int lds_base3(int lane) { return 4 * lane + 4 * (lane >> 4); }
+
+// Keeps loads issued above it ahead of the arithmetic below it.
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
+// Pass 1 uses the compile-time twiddles (Pass1Tw); twv is psi_rev (psi^bitrev(k) in the
+// field's constant form) for the per-lane reads of passes 2 and 3, normally a copy in
+// LDS. Each stage's twiddles are requested one stage ahead of their use.
 template <class F, int E>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
-                                             const typename DF<F>::W* __restrict__ tw,
                                              const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
   constexpr int LOGE = Log2<E>::value;
+  constexpr int V = E / 16;
   static_assert(LOGE > 0, "E must be 8, 16 or 32");
+  const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
   // pass 1
 #pragma unroll
   for (int s = 0; s < LOGE; s++) {
@@ -214,120 +230,144 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
     }
   }
 #pragma unroll
-  for (int e = 0; e < E; e++) lds[lds_pad(lane + 64 * e)] = x[e];
+  for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
   wave_sync();
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
-    const int u = lane + 64 * v;
-    const int base = 64 * (u >> 2) + (u & 3);
+  for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[lds_pad(base + 4 * k)];
+    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
   }
+  // pass 2: block b = (lane + 64v) >> 2, stage s uses twv[(E << s) + (b << s) + j], j < 2^s
+  W t2[V][4][8];
+  t2[0][0][0] = twv[E + (lane >> 2)];
   wave_sync();
-  // pass 2
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
+  for (int v = 0; v < V; v++) {
     const int b = (lane + 64 * v) >> 2;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      const int m = E << s;
+      if (s < 3) {
+#pragma unroll
+        for (int j = 0; j < (2 << s); j++) t2[v][s + 1][j] = twv[(E << (s + 1)) + (b << (s + 1)) + j];
+      } else if (v + 1 < V) {
+        t2[v + 1][0][0] = twv[E + ((lane + 64 * (v + 1)) >> 2)];
+      }
+      sched_fence();
       const int half = 8 >> s;
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
-        Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], twv[m + (b << s) + k0 / (2 * half)],
-                       x[16 * v + k1], x[16 * v + k1 + half], twv[m + (b << s) + k1 / (2 * half)]);
+        Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], t2[v][s][k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], t2[v][s][k1 / (2 * half)]);
       }
+      sched_fence();
     }
   }
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
-    const int u = lane + 64 * v;
-    const int base = 64 * (u >> 2) + (u & 3);
+  for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[lds_pad(base + 4 * k)] = x[16 * v + k];
+    for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
   }
   wave_sync();
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
-    const int p = 4 * (lane + 64 * q);
 #pragma unroll
-    for (int r = 0; r < 4; r++) x[4 * q + r] = lds[lds_pad(p) + r];
+    for (int r = 0; r < 4; r++) x[4 * q + r] = lds[b3 + 272 * q + r];
+  }
+  // pass 3 twiddles, requested behind the exchange reads
+  W t3[E / 4][3];
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int c = lane + 64 * q;
+    t3[q][0] = twv[16 * E + c];
+    t3[q][1] = twv[32 * E + 2 * c];
+    t3[q][2] = twv[32 * E + 2 * c + 1];
   }
   wave_sync();
   // pass 3
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
-    const int c = lane + 64 * q;
-    const W w0 = twv[16 * E + c];
-    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
-    const W w1 = twv[32 * E + 2 * c];
-    const W w2 = twv[32 * E + 2 * c + 1];
-    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
+    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
+    Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
   }
 }
 
 // Inverse transform without the 1/N factor. In: slot layout of ntt_fwd_wave.
-// Out: x[e] = position lane + 64e, canonical. itw = psi^-bitrev(k).
+// Out: x[e] = position lane + 64e, canonical. itwv = psi^-bitrev(k) for passes 3 and 2
+// (normally an LDS copy); pass 1 uses the compile-time twiddles.
 template <class F, int E>
 __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
-                                             const typename DF<F>::W* __restrict__ itw,
                                              const typename DF<F>::W* itwv, int lane) {
   using W = typename DF<F>::W;
   constexpr int LOGE = Log2<E>::value;
+  constexpr int V = E / 16;
+  const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
+  W t3[E / 4][3];
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    const int c = lane + 64 * q;
+    t3[q][0] = itwv[16 * E + c];
+    t3[q][1] = itwv[32 * E + 2 * c];
+    t3[q][2] = itwv[32 * E + 2 * c + 1];
+  }
+  sched_fence();
 #pragma unroll
   for (int e = 0; e < E; e++) x[e] = Bfly<F>::inv_in(x[e]);
   // pass 3
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
-    const int c = lane + 64 * q;
-    const W w1 = itwv[32 * E + 2 * c];
-    const W w2 = itwv[32 * E + 2 * c + 1];
-    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], w1, x[4 * q + 2], x[4 * q + 3], w2);
-    const W w0 = itwv[16 * E + c];
-    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], w0, x[4 * q + 1], x[4 * q + 3], w0);
+    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
+    Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
   }
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
-    const int p = 4 * (lane + 64 * q);
 #pragma unroll
-    for (int r = 0; r < 4; r++) lds[lds_pad(p) + r] = x[4 * q + r];
+    for (int r = 0; r < 4; r++) lds[b3 + 272 * q + r] = x[4 * q + r];
   }
   wave_sync();
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
-    const int u = lane + 64 * v;
-    const int base = 64 * (u >> 2) + (u & 3);
+  for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[lds_pad(base + 4 * k)];
+    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
   }
-  wave_sync();
-  // pass 2
+  // pass 2, stages 3..0; stage s uses itwv[(E << s) + (b << s) + j], j < 2^s
+  W t2[V][4][8];
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
+  for (int j = 0; j < 8; j++) t2[0][3][j] = itwv[(E << 3) + ((lane >> 2) << 3) + j];
+  wave_sync();
+#pragma unroll
+  for (int v = 0; v < V; v++) {
     const int b = (lane + 64 * v) >> 2;
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
-      const int m = E << s;
+      if (s > 0) {
+#pragma unroll
+        for (int j = 0; j < (1 << (s - 1)); j++)
+          t2[v][s - 1][j] = itwv[(E << (s - 1)) + (b << (s - 1)) + j];
+      } else if (v + 1 < V) {
+        const int bn = (lane + 64 * (v + 1)) >> 2;
+#pragma unroll
+        for (int j = 0; j < 8; j++) t2[v + 1][3][j] = itwv[(E << 3) + (bn << 3) + j];
+      }
+      sched_fence();
       const int half = 8 >> s;
 #pragma unroll
       for (int p = 0; p < 4; p++) {
         const int k0 = bf_lo(2 * p, half), k1 = bf_lo(2 * p + 1, half);
-        Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], itwv[m + (b << s) + k0 / (2 * half)],
-                       x[16 * v + k1], x[16 * v + k1 + half], itwv[m + (b << s) + k1 / (2 * half)]);
+        Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], t2[v][s][k0 / (2 * half)],
+                       x[16 * v + k1], x[16 * v + k1 + half], t2[v][s][k1 / (2 * half)]);
       }
+      sched_fence();
     }
   }
 #pragma unroll
-  for (int v = 0; v < E / 16; v++) {
-    const int u = lane + 64 * v;
-    const int base = 64 * (u >> 2) + (u & 3);
+  for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[lds_pad(base + 4 * k)] = x[16 * v + k];
+    for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
   }
   wave_sync();
 #pragma unroll
-  for (int e = 0; e < E; e++) x[e] = lds[lds_pad(lane + 64 * e)];
+  for (int e = 0; e < E; e++) x[e] = lds[b1 + 68 * e];
   wave_sync();
   // pass 1
 #pragma unroll
