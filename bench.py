@@ -57,6 +57,62 @@ def profiled_traffic(set_name: str, gates: int):
     return (int(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
+def microbench(L, check, hb, reps: int = 3) -> dict:
+    """Kernel microbenchmarks on Goldilocks: the MAC peak of this build (modmul/s), the
+    negacyclic NTT over 4096 polynomials for N = 1024 and 2048 (forward and inverse,
+    bytes = read + write of every word), and the external product of 4096 RLWEs with one
+    RGSW of the Zama shape (B = 2^5, 13 digits). Times are HIP events on the launch stream."""
+    P = (1 << 64) - (1 << 32) + 1
+    out = {}
+    mps, ms = C.c_double(), C.c_double()
+    check(L.hb_modmul_peak(8192, 1024, C.byref(mps), C.byref(ms)))
+    out["modmul_peak_per_s"] = mps.value
+    out["modmul_peak_ms"] = round(ms.value, 3)
+    rng = np.random.default_rng([2, 2])
+    count = 4096
+
+    def dev(a):
+        p = C.c_void_p()
+        check(L.hb_dev_alloc(a.nbytes, C.byref(p)))
+        check(L.hb_memcpy_h2d(p, a.ctypes.data, a.nbytes))
+        return p
+
+    for log_n in (10, 11):
+        N = 1 << log_n
+        d = dev(rng.integers(0, P, size=(count, N), dtype=np.uint64))
+        for inv in (0, 1):
+            check(L.hb_ntt_dev(1, inv, log_n, d, count, None))
+            check(L.hb_sync())
+            t = []
+            for _ in range(reps):
+                check(L.hb_ntt_dev(1, inv, log_n, d, count, None))
+                t.append(hb.last_kernel_ms())
+            kms = float(np.median(t))
+            out[f"ntt_{'inv' if inv else 'fwd'}_N{N}"] = {
+                "polys": count, "ms": round(kms, 4),
+                "GB_per_s": round(2 * 8 * N * count / (kms * 1e-3) / 1e9, 1)}
+        L.hb_dev_free(d)
+    N, bits, length = 1024, 5, 13
+    rgsw = dev(rng.integers(0, P, size=2 * length * 2 * N, dtype=np.uint64))
+    rl = dev(rng.integers(0, P, size=(count, 2 * N), dtype=np.uint64))
+    check(L.hb_external_product_dev(1, 10, bits, length, rgsw, rl, count, None))
+    check(L.hb_sync())
+    t = []
+    for _ in range(reps):
+        check(L.hb_external_product_dev(1, 10, bits, length, rgsw, rl, count, None))
+        t.append(hb.last_kernel_ms())
+    kms = float(np.median(t))
+    alg = count * 4 * N * 8 + 2 * length * 2 * N * 8
+    mm = count * ((2 * length + 2) * (N // 2) * 10 + 4 * length * N)
+    out["external_product_zama_shape"] = {
+        "rlwes": count, "ms": round(kms, 4),
+        "GB_per_s": round(alg / (kms * 1e-3) / 1e9, 1),
+        "modmul_per_s": round(mm / (kms * 1e-3), 1)}
+    L.hb_dev_free(rgsw)
+    L.hb_dev_free(rl)
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -83,6 +139,12 @@ def main() -> int:
                          "empty to skip")
     ap.add_argument("--serving-seconds", type=float, default=8.0)
     ap.add_argument("--serving-clients", type=int, default=8)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the gates-per-workgroup autotuner before the batched leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager launches instead of replays of the captured hipGraph")
+    ap.add_argument("--no-micro", action="store_true",
+                    help="skip the NTT / external-product / MAC-peak microbenchmarks")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the batched leg, copy a device buffer of this size once "
                          "(a reference kernel for scaling HBM byte counters)")
@@ -141,8 +203,25 @@ def main() -> int:
     def launch(count, a=d0, b=d1, o=dout):
         check(L.hb_gate_batch_dev(ev._h, 0, count, a, b, None, o, None))
 
+    tune = None
+    if not args.no_autotune:
+        tune = ev.autotune(probe_gates=min(G, 1024))
+        log(f"autotune: {tune['gates_per_workgroup']} gates per workgroup "
+            f"(probe ms {', '.join(f'{k}:{v:.1f}' for k, v in tune['ms'].items())})")
+
+    graph = None
+    if not args.no_graph:
+        graph = C.c_void_p()
+        check(L.hb_batch_graph_create(ev._h, 0, G, d0, d1, None, dout, C.byref(graph)))
+
+    def launch_batch():
+        if graph is not None:
+            check(L.hb_batch_graph_launch(graph, None))
+        else:
+            launch(G)
+
     for _ in range(args.warmup):
-        launch(G)
+        launch_batch()
         check(L.hb_sync())
     log(f"warmup done ({args.warmup} launches)")
 
@@ -150,7 +229,7 @@ def main() -> int:
     check(L.hb_sync())
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        launch(G)
+        launch_batch()
         kernel_ms.append(hb.last_kernel_ms())
     check(L.hb_sync())
     t1 = time.perf_counter()
@@ -220,6 +299,11 @@ def main() -> int:
                               "launch done", "sweep": sweep}
         del clients
 
+    micro = None
+    if not args.no_micro:
+        micro = microbench(L, check, hb)
+        log(f"microbench: {micro}")
+
     wb = 8 if P.is_goldilocks else 4
     key_bytes = (bsk.size + ksk.size) * wb
     io_bytes = G * 3 * n1 * 8
@@ -265,11 +349,14 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "u64" if P.is_goldilocks else "u32",
         "data": "synthetic (seeded keys and uniform random input bits)",
-        "config": {"workload": f"batched NAND bootstrap, {SET_NAMES[set_id]}, "
-                               f"{G} gates per launch, keys resident in HBM",
+        "config": {"workload": f"batched NAND bootstrap, {SET_NAMES[set_id]}, {G} gates per "
+                               f"{'eager' if graph is None else 'hipGraph-replayed'} launch, "
+                               "keys resident in HBM",
                    "parameter_set": SET_NAMES[set_id], "gates_per_launch": G,
                    "n": P.n, "N": P.N, "br_bits": P.br_bits, "br_len": P.br_len,
                    "ks_bits": P.ks_bits, "ks_len": P.ks_len},
+        "launch": "hipGraph replay" if graph is not None else "eager",
+        "autotune": tune,
         "batched_vs_plain": round(value / plain["value"], 1) if plain else None,
         "plain": plain,
         "batch_latency_ms": {"p50": round(float(np.percentile(kernel_ms, 50)), 3),
@@ -281,13 +368,21 @@ def main() -> int:
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": traffic_alg},
         "valu": {"modmul_per_gate": mm,
-                 "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1)},
+                 "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1),
+                 "peak_modmul_per_s": round(micro["modmul_peak_per_s"], 1) if micro else None,
+                 "frac": round(mm * G / (kms * 1e-3) / micro["modmul_peak_per_s"], 4)
+                 if micro else None,
+                 "peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
+                                "8192 wavefronts"},
+        "microbench": micro,
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
                     "oracle_gates": args.check},
         "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if graph is not None:
+        L.hb_batch_graph_destroy(graph)
     for p in (d0, d1, dout):
         L.hb_dev_free(p)
     return 0

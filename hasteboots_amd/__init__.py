@@ -9,6 +9,7 @@ from .tfhe import (  # noqa: F401
     SET_BABYBEAR,
     SET_GOLDILOCKS,
     SET_ZAMA,
+    BatchGraph,
     Decryptor,
     Encryptor,
     Evaluator,
@@ -19,4 +20,6 @@ from .tfhe import (  # noqa: F401
     device_count,
     last_kernel_ms,
     mixed_batch,
+    set_tuned_config,
+    tuned_config,
 )

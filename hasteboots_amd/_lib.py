@@ -49,6 +49,11 @@ class HbServerStats(C.Structure):
                 ("busy_ms", C.c_double), ("gate_p50_ms", C.c_double), ("gate_p99_ms", C.c_double)]
 
 
+class HbTuneResult(C.Structure):
+    _fields_ = [("gates_per_workgroup", C.c_int32), ("pad_", C.c_int32),
+                ("probe_gates", C.c_uint64), ("ms", C.c_double * 3)]
+
+
 class HbError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"hb error {code}: {msg}")
@@ -76,6 +81,13 @@ _PROTOS = {
     "hb_gate_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]),
     "hb_gate_batch_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hb_batch_graph_create": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "hb_batch_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "hb_batch_graph_destroy": (C.c_int, [C.c_void_p]),
+    "hb_autotune": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(HbTuneResult)]),
+    "hb_tuned_config": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hb_set_tuned_config": (C.c_int, [C.c_int, C.c_int]),
     "hb_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "hb_server_destroy": (C.c_int, [C.c_void_p]),
     "hb_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(HbCircuit), u64p,
@@ -98,6 +110,8 @@ _PROTOS = {
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "hb_external_product_dev": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "hb_modmul_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
     "hb_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
     "hb_dev_free": (C.c_int, [C.c_void_p]),
     "hb_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),

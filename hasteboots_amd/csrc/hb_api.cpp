@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -53,6 +54,39 @@ uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
 
 }  // namespace
 
+namespace {
+
+// Autotune cache: text lines "<set> <gates per workgroup>", at the path named by the
+// HB_AUTOTUNE_CACHE environment variable (no file, no cache).
+const char* tune_cache_path() { return std::getenv("HB_AUTOTUNE_CACHE"); }
+
+void load_tune_cache(Engine& E) {
+  const char* path = tune_cache_path();
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  int set, gpw;
+  while (std::fscanf(f, "%d %d", &set, &gpw) == 2) {
+    if (set >= 0 && set < 3 && (gpw == 1 || gpw == 2 || gpw == 4)) {
+      E.gpw[set] = gpw;
+      E.tuned[set] = true;
+    }
+  }
+  std::fclose(f);
+}
+
+void save_tune_cache(const Engine& E) {
+  const char* path = tune_cache_path();
+  if (!path) return;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return;
+  for (int set = 0; set < 3; set++)
+    if (E.tuned[set]) std::fprintf(f, "%d %d\n", set, E.gpw[set]);
+  std::fclose(f);
+}
+
+}  // namespace
+
 int hb::ensure_engine(int device) {
   Engine& E = engine();
   if (E.device >= 0) {
@@ -95,6 +129,7 @@ int hb::ensure_engine(int device) {
   }
   E.tables.ready = true;
   E.device = device;
+  load_tune_cache(E);
   return HB_OK;
 }
 
@@ -225,7 +260,7 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   if (rc) return rc;
   BootArgs a = make_args(ek, gate, count, c0, c1, c2, lut_dev, out, ek->acc_dev);
   HIP_TRY(hipEventRecord(E.ev0, s));
-  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, s));
+  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, E.gpw[p.set], s));
   HIP_TRY(hipEventRecord(E.ev1, s));
   return HB_OK;
 }
@@ -561,7 +596,7 @@ int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const
   BootArgs a = make_args(ek, gate, count, d0, c1s ? d1 : nullptr, c2s ? d2 : nullptr,
                          lut ? (const uint64_t*)ek->lut_dev : nullptr, dout, ek->acc_dev);
   a.trace = tr;
-  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, E.stream) != hipSuccess ||
+  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, 1, E.stream) != hipSuccess ||
       hipEventRecord(E.ev1, E.stream) != hipSuccess)
     return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
   std::vector<uint8_t> host(words * wb);
@@ -740,3 +775,169 @@ int hb_last_kernel_ms(double* out) {
 const char* hb_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"
+
+// ----------------------------------------------------------------------------
+// Captured batch launches and the autotuner.
+// ----------------------------------------------------------------------------
+struct hb_batch_graph {
+  hb_ek* ek = nullptr;
+  void* acc = nullptr;  // private accumulator scratch (count * 2N words)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t count = 0;
+};
+
+int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s,
+                          const uint64_t* c1s, const uint64_t* c2s, uint64_t* outs,
+                          hb_batch_graph_t** out) {
+  if (!ek || !c0s || !c1s || !outs || !out) return fail(HB_ERR_ARG, "null argument");
+  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (gate == GATE_MAJORITY && !c2s) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (count == 0 || count > 0x7FFFFFFF) return fail(HB_ERR_ARG, "bad count");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  Engine& E = engine();
+  std::unique_ptr<hb_batch_graph> g(new hb_batch_graph);
+  g->ek = ek;
+  g->count = count;
+  const size_t wb = ek->p.is_goldilocks ? 8 : 4;
+  HIP_TRY(hipMalloc(&g->acc, count * 2 * ek->p.N * wb));
+  BootArgs a = make_args(ek, gate, count, c0s, c1s, c2s, nullptr, outs, g->acc);
+  hipStream_t cs;
+  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    hipError_t le = launch_bootstrap(a, ek->p.is_goldilocks, E.gpw[ek->p.set], cs);
+    e = hipStreamEndCapture(cs, &g->graph);
+    if (le != hipSuccess) e = le;
+  }
+  (void)hipStreamDestroy(cs);
+  if (e != hipSuccess) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->acc);
+    return fail(HB_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->acc);
+    return fail(HB_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+  }
+  *out = g.release();
+  return HB_OK;
+}
+
+int hb_batch_graph_launch(hb_batch_graph_t* g, void* stream) {
+  if (!g) return fail(HB_ERR_ARG, "null graph");
+  Engine& E = engine();
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  HIP_TRY(hipEventRecord(E.ev0, s));
+  HIP_TRY(hipGraphLaunch(g->exec, s));
+  HIP_TRY(hipEventRecord(E.ev1, s));
+  return HB_OK;
+}
+
+int hb_batch_graph_destroy(hb_batch_graph_t* g) {
+  if (!g) return HB_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->acc) (void)hipFree(g->acc);
+  delete g;
+  return HB_OK;
+}
+
+int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out) {
+  if (!ek || !out) return fail(HB_ERR_ARG, "null argument");
+  if (probe_gates == 0 || probe_gates > 65536) return fail(HB_ERR_ARG, "probe_gates out of range");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  Engine& E = engine();
+  std::lock_guard<std::mutex> lk(ek->mu);
+  rc = ensure_io(ek, probe_gates);
+  if (rc) return rc;
+  rc = ensure_acc(ek, probe_gates);
+  if (rc) return rc;
+  const size_t n1 = ek->p.n + 1;
+  // Uniform canonical words as inputs: every blind-rotation round is active, as for
+  // real ciphertexts (a zero input would skip all rounds).
+  {
+    std::vector<uint64_t> h(probe_gates * 2 * n1);
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (auto& w : h) {
+      x ^= x << 13, x ^= x >> 7, x ^= x << 17;
+      w = x % ek->p.q;
+    }
+    HIP_TRY(hipMemcpy(ek->io_dev, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  }
+  uint64_t* c0 = ek->io_dev;
+  uint64_t* c1 = c0 + probe_gates * n1;
+  uint64_t* o = c1 + probe_gates * n1;
+  BootArgs a = make_args(ek, GATE_NAND, probe_gates, c0, c1, nullptr, nullptr, o, ek->acc_dev);
+  static const int kGpw[3] = {1, 2, 4};
+  double best = 1e300;
+  int best_gpw = 1;
+  for (int v = 0; v < 3; v++) {
+    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, kGpw[v], E.stream));  // warm-up
+    HIP_TRY(hipEventRecord(E.ev0, E.stream));
+    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, kGpw[v], E.stream));
+    HIP_TRY(hipEventRecord(E.ev1, E.stream));
+    HIP_TRY(hipEventSynchronize(E.ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+    out->ms[v] = ms;
+    if (ms < best) {
+      best = ms;
+      best_gpw = kGpw[v];
+    }
+  }
+  E.gpw[ek->p.set] = best_gpw;
+  E.tuned[ek->p.set] = true;
+  save_tune_cache(E);
+  out->gates_per_workgroup = best_gpw;
+  out->probe_gates = probe_gates;
+  return HB_OK;
+}
+
+int hb_tuned_config(int set, int* gates_per_workgroup, int* tuned) {
+  if (set < 0 || set > 2 || !gates_per_workgroup) return fail(HB_ERR_ARG, "bad argument");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  *gates_per_workgroup = engine().gpw[set];
+  if (tuned) *tuned = engine().tuned[set] ? 1 : 0;
+  return HB_OK;
+}
+
+int hb_set_tuned_config(int set, int gates_per_workgroup) {
+  if (set < 0 || set > 2) return fail(HB_ERR_ARG, "bad parameter set %d", set);
+  if (gates_per_workgroup != 1 && gates_per_workgroup != 2 && gates_per_workgroup != 4)
+    return fail(HB_ERR_ARG, "gates per workgroup must be 1, 2 or 4");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  engine().gpw[set] = gates_per_workgroup;
+  engine().tuned[set] = true;
+  return HB_OK;
+}
+
+int hb_modmul_peak(uint32_t waves, uint32_t iters, double* modmul_per_s, double* ms_out) {
+  if (!modmul_per_s || waves == 0 || iters == 0 || waves > (1u << 20))
+    return fail(HB_ERR_ARG, "bad modmul peak arguments");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  Engine& E = engine();
+  uint64_t* out = nullptr;
+  HIP_TRY(hipMalloc(&out, (size_t)waves * 64 * 8));
+  hipError_t e = launch_modmul_peak(out, waves, iters, 7, E.stream);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(E.ev0, E.stream);
+  if (e == hipSuccess) e = launch_modmul_peak(out, waves, iters, 11, E.stream);
+  if (e == hipSuccess) e = hipEventRecord(E.ev1, E.stream);
+  if (e == hipSuccess) e = hipEventSynchronize(E.ev1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, E.ev0, E.ev1);
+  (void)hipFree(out);
+  if (e != hipSuccess) return fail(HB_ERR_DEVICE, "modmul peak: %s", hipGetErrorString(e));
+  E.last_ms = ms;
+  // per lane and iteration: 4 gl_mac2 = 8 products
+  *modmul_per_s = (double)waves * 64.0 * iters * 8.0 / (ms * 1e-3);
+  if (ms_out) *ms_out = ms;
+  return HB_OK;
+}

@@ -45,6 +45,10 @@ struct Engine {
   size_t mixed_parts_cap = 0;
   uint64_t* mixed_io = nullptr;
   size_t mixed_io_words = 0;
+  // Autotuned gates (wavefronts) per workgroup of the batched kernel, per parameter
+  // set (HB_SET_*); 1 until hb_autotune runs or a cache file supplies a value.
+  int gpw[3] = {1, 1, 1};
+  bool tuned[3] = {false, false, false};
 };
 
 Engine& engine();

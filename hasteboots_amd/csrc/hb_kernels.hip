@@ -171,8 +171,19 @@ __device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
   return v;
 }
 
+// Orders one wave's accumulator (global) and LDS accesses between steps of a gate:
+// workgroup-scope release/acquire fences (the wave's own stores complete and become
+// visible to its later loads) without a barrier, so the waves of a workgroup, one gate
+// each, never wait for one another.
+__device__ __forceinline__ void gate_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ----------------------------------------------------------------------------
-// Fused bootstrap: one 64-thread workgroup (one wavefront) per gate.
+// Fused bootstrap: one wavefront per gate, 1, 2 or 4 gates per workgroup (the
+// workgroup shares the LDS twiddle tables).
 // EvaluationKey::bootstrap (tfhe/src/bfhe/evaluate.rs:42-60):
 //   lwe_modulus_switch → BinaryBlindRotationKey::blind_rotate (binary.rs:28-104)
 //   → acc.b[0] += Q/8 → KeySwitchingRLWEKey::key_switch_for_rlwe
@@ -186,7 +197,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const u32 n = A.n, n1 = A.n + 1;
   const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ itw = static_cast<const W*>(A.tw_inv);
@@ -242,7 +253,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
     accA[k] = 0;
     accB[k] = val;
   }
-  __syncthreads();
+  gate_sync();
 
   // 3. blind rotation: for a_i != 0, ACC += ExtProd((X^a_i - 1) ACC, RGSW(s_i)).
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
@@ -259,7 +270,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
           outA[k] = accA[k];
           outB[k] = accB[k];
         }
-        __syncthreads();
+        gate_sync();
       }
       continue;
     }
@@ -310,7 +321,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
       const int k = lane + 64 * e;
       outB[k] = F::add(accB[k], pb[e]);
     }
-    __syncthreads();
+    gate_sync();
   }
 
   // 4. acc.b[0] += Q/8 (evaluate.rs:52) and key switching.
@@ -361,14 +372,17 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
   __syncthreads();
 }
 
-template <class F>
-__global__ __launch_bounds__(64) void bootstrap_kernel(BootArgs A) {
+template <class F, int GPW>
+__global__ __launch_bounds__(64 * GPW) void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
-  __shared__ W lds[1024 + 64];
-  __shared__ unsigned short ms[1024];
+  __shared__ W lds[GPW][1024 + 64];
+  __shared__ unsigned short ms[GPW][1024];
   __shared__ W twl[2][1024];
-  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64);
-  bootstrap_gate<F>(A, blockIdx.x, lds, ms, twl[0], twl[1]);
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * GPW);
+  const int w = threadIdx.x >> 6;
+  const u32 g = blockIdx.x * GPW + w;
+  if (g >= A.count) return;  // after the only workgroup barrier
+  bootstrap_gate<F>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -484,15 +498,53 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
 }
 
 // ----------------------------------------------------------------------------
+// Microbench: peak rate of the Goldilocks multiply-accumulate of the MAC stage (per
+// lane and gl_mac2: two Montgomery products and two modular adds) on register-resident
+// operands, 4 independent accumulator pairs per lane. One word per lane is stored so
+// nothing is elided.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void modmul_peak_kernel(u64* __restrict__ out, u32 iters, u64 seed) {
+  const u64 tid = (u64)blockIdx.x * 64 + threadIdx.x;
+  u64 x[4], pa[4], pb[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    x[i] = Gl::canon(seed * (tid + 1) + 0x9E3779B97F4A7C15ull * (i + 1));
+    pa[i] = Gl::canon(x[i] ^ 0x5555AAAA5555AAAAull);
+    pb[i] = Gl::canon(x[i] * 3);
+  }
+  const u64 ka = Gl::canon(seed ^ 0x0123456789ABCDEFull), kb = Gl::canon(seed + 0xFEDCBA98ull);
+  for (u32 it = 0; it < iters; it++) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) gl_mac2(pa[i], pb[i], x[i], ka, kb);
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[i] = pa[i];
+  }
+  out[tid] = pa[0] ^ pa[1] ^ pa[2] ^ pa[3] ^ pb[0] ^ pb[1] ^ pb[2] ^ pb[3];
+}
+
+// ----------------------------------------------------------------------------
 // Launchers
 // ----------------------------------------------------------------------------
-hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s) {
-  dim3 grid(a.count), block(64);
-  if (is_goldilocks)
-    hipLaunchKernelGGL(bootstrap_kernel<Gl>, grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL(bootstrap_kernel<Bb>, grid, block, 0, s, a);
+template <class F>
+static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, hipStream_t s) {
+  switch (gpw) {
+    case 1:
+      hipLaunchKernelGGL((bootstrap_kernel<F, 1>), dim3(a.count), dim3(64), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((bootstrap_kernel<F, 2>), dim3((a.count + 1) / 2), dim3(128), 0, s, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL((bootstrap_kernel<F, 4>), dim3((a.count + 3) / 4), dim3(256), 0, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, hipStream_t s) {
+  return is_goldilocks ? launch_bootstrap_f<Gl>(a, gpw, s) : launch_bootstrap_f<Bb>(a, gpw, s);
 }
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {
@@ -518,6 +570,11 @@ hipError_t launch_ntt(int is_goldilocks, int inverse, u32 log_n, u64* data, size
     else
       return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_modmul_peak(u64* out, u32 waves, u32 iters, u64 seed, hipStream_t s) {
+  hipLaunchKernelGGL(modmul_peak_kernel, dim3(waves), dim3(64), 0, s, out, iters, seed);
   return hipGetLastError();
 }
 

@@ -42,12 +42,17 @@ struct BootArgs {
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
 };
 
-hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, hipStream_t s);
+// gpw: gates (wavefronts) per workgroup, 1, 2 or 4.
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
                       const void* tw, uint64_t ninv, hipStream_t s);
+// Goldilocks MAC peak microbench: `waves` wavefronts, `iters` x 4 gl_mac2 per lane;
+// out holds waves*64 words.
+hipError_t launch_modmul_peak(uint64_t* out, uint32_t waves, uint32_t iters, uint64_t seed,
+                              hipStream_t s);
 hipError_t launch_extprod(int is_goldilocks, uint64_t* rlwe, size_t count, const void* rgsw,
                           uint32_t len, uint32_t bits, const void* tw, const void* itw,
                           hipStream_t s);

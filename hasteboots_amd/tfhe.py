@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import HbParams, check, lib, ptr
+from ._lib import HbTuneResult, HbParams, check, lib, ptr
 
 SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA = 0, 1, 2
 GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6}
@@ -278,6 +278,19 @@ class Evaluator:
     def nand_batch(self, c0s: np.ndarray, c1s: np.ndarray) -> np.ndarray:
         return self.gate_batch("nand", c0s, c1s)
 
+    def autotune(self, probe_gates: int = 1024) -> dict:
+        """Times the 1/2/4 gates-per-workgroup variants of the batched kernel for this
+        parameter set and keeps the fastest (hb_autotune)."""
+        r = HbTuneResult()
+        check(lib().hb_autotune(self._h, probe_gates, C.byref(r)))
+        return {"gates_per_workgroup": r.gates_per_workgroup, "probe_gates": r.probe_gates,
+                "ms": {1: r.ms[0], 2: r.ms[1], 4: r.ms[2]}}
+
+    def batch_graph(self, gate: str, count: int):
+        """A captured (hipGraph) batch launch over device buffers owned by the returned
+        BatchGraph; see BatchGraph."""
+        return BatchGraph(self, gate, count)
+
     # ---- traced gates (vfhe::Evaluator, vfhe/src/bfhe/evaluate.rs:47-139) ----
     def gate_batch_traced(self, gate: str, c0s, c1s, c2s=None):
         """Gates with the traced path's semantics; returns (outputs, core traces) where the
@@ -321,6 +334,76 @@ class Evaluator:
                                     rounds.ctypes.data_as(C.POINTER(C.c_uint32)), rounds.size,
                                     ptr(rec), ptr(tail)))
         return rec, tail
+
+
+class BatchGraph:
+    """hb_batch_graph_*: `count` gates of one kind over fixed device buffers, captured once
+    as a hipGraph and replayed. set_inputs() stages host ciphertexts, launch() replays,
+    outputs() copies the results back."""
+
+    def __init__(self, ev: Evaluator, gate: str, count: int):
+        L = lib()
+        self.ev, self.count = ev, count
+        self.n1 = ev.params.n + 1
+        nb = count * self.n1 * 8
+        self._bufs = []
+        for _ in range(4 if gate == "majority" else 3):
+            p = C.c_void_p()
+            check(L.hb_dev_alloc(nb, C.byref(p)))
+            self._bufs.append(p)
+        c2 = self._bufs[3] if gate == "majority" else None
+        g = C.c_void_p()
+        check(L.hb_batch_graph_create(ev._h, GATES[gate], count, self._bufs[0], self._bufs[1],
+                                      c2, self._bufs[2], C.byref(g)))
+        self._g = g
+
+    def set_inputs(self, c0s, c1s, c2s=None):
+        L = lib()
+        for buf, a in ((self._bufs[0], c0s), (self._bufs[1], c1s),
+                       (self._bufs[3] if c2s is not None else None, c2s)):
+            if a is None:
+                continue
+            a = np.ascontiguousarray(a, np.uint64)
+            assert a.shape == (self.count, self.n1)
+            check(L.hb_memcpy_h2d(buf, a.ctypes.data, a.nbytes))
+
+    def launch(self, stream=None):
+        check(lib().hb_batch_graph_launch(self._g, stream))
+
+    def outputs(self) -> np.ndarray:
+        out = np.zeros((self.count, self.n1), dtype=np.uint64)
+        check(lib().hb_memcpy_d2h(out.ctypes.data, self._bufs[2], out.nbytes))
+        return out
+
+    @property
+    def device_buffers(self):
+        return self._bufs
+
+    def close(self):
+        L = lib()
+        if getattr(self, "_g", None):
+            L.hb_batch_graph_destroy(self._g)
+            self._g = None
+        for b in getattr(self, "_bufs", []):
+            L.hb_dev_free(b)
+        self._bufs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tuned_config(set_id: int) -> tuple[int, bool]:
+    """(gates per workgroup, whether it was tuned) for a parameter set."""
+    g, t = C.c_int(), C.c_int()
+    check(lib().hb_tuned_config(set_id, C.byref(g), C.byref(t)))
+    return g.value, bool(t.value)
+
+
+def set_tuned_config(set_id: int, gates_per_workgroup: int) -> None:
+    check(lib().hb_set_tuned_config(set_id, gates_per_workgroup))
 
 
 def mixed_batch(parts) -> list[np.ndarray]:

@@ -125,6 +125,34 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_d
                       const uint64_t* c1s_dev, const uint64_t* c2s_dev, uint64_t* outs_dev,
                       void* stream);
 
+/* Captured batch launch: hb_gate_batch_dev for fixed device buffers recorded once as a
+ * hipGraph (with its own accumulator scratch) and replayed by hb_batch_graph_launch
+ * without per-call host work. Inputs may be rewritten between launches; replays on one
+ * graph must not overlap. No reference counterpart (the reference has no device launch). */
+typedef struct hb_batch_graph hb_batch_graph_t;
+int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_dev,
+                          const uint64_t* c1s_dev, const uint64_t* c2s_dev, uint64_t* outs_dev,
+                          hb_batch_graph_t** out);
+int hb_batch_graph_launch(hb_batch_graph_t* graph, void* stream);
+int hb_batch_graph_destroy(hb_batch_graph_t* graph);
+
+/* Autotuner for the batched kernel: times the variants with 1, 2 and 4 gates
+ * (wavefronts) per workgroup on `probe_gates` gates with this key's parameter set, keeps
+ * the fastest for that set (used by every later batch launch of the set) and, when the
+ * environment variable HB_AUTOTUNE_CACHE names a file, stores the choice there; a new
+ * process reads it back. No reference counterpart. */
+typedef struct {
+  int gates_per_workgroup;
+  int pad_;
+  uint64_t probe_gates;
+  double ms[3]; /* kernel time of the 1, 2 and 4 gates-per-workgroup variants */
+} hb_tune_result_t;
+int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out);
+/* Current choice for a parameter set; *tuned = 1 if it came from hb_autotune or the cache. */
+int hb_tuned_config(int set, int* gates_per_workgroup, int* tuned);
+/* Pins the variant for a parameter set (1, 2 or 4 gates per workgroup). */
+int hb_set_tuned_config(int set, int gates_per_workgroup);
+
 /* Mixed-parameter batch: several parts, each with its own evaluation key (any of the
  * three parameter sets) and gate kind, bootstrapped in ONE kernel launch. No reference
  * counterpart (the reference evaluates one gate per call). Heavier parameter sets are
@@ -236,6 +264,12 @@ int hb_ntt_dev(int field, int inverse, uint32_t log_n, uint64_t* data_dev, size_
 int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t len,
                             const uint64_t* rgsw_dev, uint64_t* rlwe_dev, size_t count,
                             void* stream);
+
+/* Peak Goldilocks multiply-accumulate rate of this build on the device: `waves`
+ * wavefronts each run iters x 4 register-resident MAC pairs (the MAC stage's sequence:
+ * Montgomery product + modular add). *modmul_per_s counts products. Measurement only;
+ * no reference counterpart. */
+int hb_modmul_peak(uint32_t waves, uint32_t iters, double* modmul_per_s, double* ms);
 
 /* Device memory helpers so callers without a HIP runtime of their own can stage data. */
 int hb_dev_alloc(size_t bytes, void** out);
