@@ -155,7 +155,7 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
-    ntt_fwd_wave<F, E>(x, lds, twv, lane);
+    ntt_fwd_wave<F, E, true>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
   }
 }

@@ -208,7 +208,11 @@ __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0
 // Pass 1 uses the compile-time twiddles (Pass1Tw); twv is psi_rev (psi^bitrev(k) in the
 // field's constant form) for the per-lane reads of passes 2 and 3, normally a copy in
 // LDS. Each stage's twiddles are requested one stage ahead of their use.
-template <class F, int E>
+// With SMALL (Goldilocks only), every input word is below 2^16 (gadget digits): stage 0,
+// whose twiddle is psi^(N/2) = 2^48, is then exact without reduction:
+//   x + 2^48 y = (2^16 y) : x   and   x - 2^48 y + P = ~(2^16 y) : (x + 1)
+// as (high 32 bits : low 32 bits); both are valid words in [0, 2^64).
+template <class F, int E, bool SMALL = false>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
                                              const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
@@ -216,9 +220,21 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
   constexpr int V = E / 16;
   static_assert(LOGE > 0, "E must be 8, 16 or 32");
   const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
+  constexpr bool SMALL0 = SMALL && F::IS_GOLDILOCKS;
+  if constexpr (SMALL0) {
+    constexpr Pass1Tw<F, LOGE + 6, false> T{};
+    // Montgomery form of 2^48 is 2^112 = -2^16 mod P
+    static_assert((u64)T.w[1] == Gl::P - (1ull << 16), "stage-0 twiddle is not 2^48");
+#pragma unroll
+    for (int e = 0; e < E / 2; e++) {
+      const u32 a0 = (u32)x[e], t1 = (u32)x[e + E / 2] << 16;
+      x[e] = ((u64)t1 << 32) | a0;
+      x[e + E / 2] = ((u64)(~t1) << 32) | (a0 + 1u);
+    }
+  }
   // pass 1
 #pragma unroll
-  for (int s = 0; s < LOGE; s++) {
+  for (int s = SMALL0 ? 1 : 0; s < LOGE; s++) {
     const int m = 1 << s;
     const int half = (E / 2) >> s;
 #pragma unroll
