@@ -91,13 +91,82 @@ def montmul(c: Chain, y0: str, y1: str, w0: str, w1: str, dst: str = "A"):
     c.add(f"v_mad_u64_u32 {{W}}, {{D}}, {y1}, {w1}, {{Z}}", ["W"], [y1, w1, "Z"])
     c.add("v_mad_u64_u32 {W}, {D}, {V.hi}, 1, {W}", ["W"], ["V.hi", "W"])
     # lo = (A.lo, V.lo), hi = (W.lo, W.hi)
-    c.add("v_add_co_u32 {T1}, {S1}, {V.lo}, {A.lo}", ["T1", "S1"], ["V.lo", "A.lo"])
-    c.add("v_subb_co_u32 {T2}, {S2}, {A.lo}, {T1}, {S1}", ["T2", "S2"], ["A.lo", "T1", "S1"])
+    monty(c, "A.lo", "V.lo", "W.lo", "W.hi", dst)
+
+
+def word(sym: str) -> str:
+    """Asm text of a 32-bit word symbol: a fixed register, an operand or the constant 0."""
+    if sym == "0":
+        return "0"
+    return sym if sym.startswith("%") else "{" + sym + "}"
+
+
+def monty(c: Chain, l0: str, l1: str, h0: str, h1: str, dst: str):
+    """dst <- (lo + hi * 2^64) * 2^-64 mod P, canonical (hi < P); words may be "0".
+    a = lo + (lo << 32) (carry e); b = a - (a >> 32) - e; r = hi - b (borrow c);
+    r -= c ? EPS : 0. Writes A, T1, T2, S1, S2 after reading its inputs."""
+    w = word
+    c.add(f"v_add_co_u32 {{T1}}, {{S1}}, {w(l1)}, {w(l0)}", ["T1", "S1"], [l1, l0])
+    c.add(f"v_subb_co_u32 {{T2}}, {{S2}}, {w(l0)}, {{T1}}, {{S1}}", ["T2", "S2"], [l0, "T1", "S1"])
     c.add("v_subb_co_u32 {T1}, {S2}, {T1}, 0, {S2}", ["T1", "S2"], ["T1", "S2"])
-    c.add("v_sub_co_u32 {A.lo}, {S1}, {W.lo}, {T2}", ["A.lo", "S1"], ["W.lo", "T2"])
-    c.add("v_subb_co_u32 {A.hi}, {S1}, {W.hi}, {T1}, {S1}", ["A.hi", "S1"], ["W.hi", "T1", "S1"])
+    c.add(f"v_sub_co_u32 {{A.lo}}, {{S1}}, {w(h0)}, {{T2}}", ["A.lo", "S1"], [h0, "T2"])
+    c.add(f"v_subb_co_u32 {{A.hi}}, {{S1}}, {w(h1)}, {{T1}}, {{S1}}", ["A.hi", "S1"], [h1, "T1", "S1"])
     c.add("v_cndmask_b32 {T2}, 0, -3, {S1}", ["T2"], ["S1"])
     c.add(f"v_mad_i64_i32 {{{dst}}}, {{D}}, {{T2}}, {{%k55}}, {{A}}", [dst], ["T2", "%k55", "A"])
+
+
+def mulpow2(c: Chain, typ: str, y0: str, y1: str, dst: str):
+    """dst <- y * 2^e mod P, canonical, y in [0, 2^64) given as words y0, y1; the shift
+    amounts are the chain's immediate operands %s<tag>1, %s<tag>2.
+      A  (e = k, 1 <= k < 32; s1 = k, s2 = 32 - k): y * 2^k = W0 + W1 * 2^32 + W2 * 2^64
+         with W2 < 2^32, so t = (W0, W1) + W2 * EPS (carry c), q = t + EPS (carry c2),
+         result q if c or c2 else t.
+      BL (1 <= j < 32; s1 = j, s2 = 32 - j) and BH (32 < j < 64; s1 = j - 32, s2 = 64 - j):
+         the Montgomery reduction of the 128-bit y * 2^j, i.e. y * 2^(j - 64)."""
+    s1, s2 = "{%s" + c.tag + "1}", "{%s" + c.tag + "2}"
+    y0t, y1t = word(y0), word(y1)
+    if typ == "A":
+        c.add(f"v_lshlrev_b32 {{A.lo}}, {s1}, {y0t}", ["A.lo"], [y0])
+        c.add(f"v_lshrrev_b32 {{T1}}, {s2}, {y1t}", ["T1"], [y1])
+        c.add(f"v_alignbit_b32 {{A.hi}}, {y1t}, {y0t}, {s2}", ["A.hi"], [y1, y0])
+        c.add("v_mad_u64_u32 {A}, {S1}, {T1}, -1, {A}", ["A", "S1"], ["T1", "A"])
+        c.add("v_mad_u64_u32 {U}, {S2}, -1, 1, {A}", ["U", "S2"], ["A"])
+        c.add("s_or_b64 {S1}, {S1}, {S2}", ["S1"], ["S1", "S2"])
+        c.add("v_cndmask_b32 {A.lo}, {A.lo}, {U.lo}, {S1}", ["A.lo"], ["A.lo", "U.lo", "S1"])
+        c.add("v_cndmask_b32 {A.hi}, {A.hi}, {U.hi}, {S1}", ["A.hi"], ["A.hi", "U.hi", "S1"])
+        if dst != "A":
+            c.add(f"v_mov_b64 {{{dst}}}, {{A}}", [dst], ["A"])
+    elif typ == "BL":
+        c.add(f"v_lshlrev_b32 {{A.lo}}, {s1}, {y0t}", ["A.lo"], [y0])
+        c.add(f"v_lshrrev_b32 {{W.lo}}, {s2}, {y1t}", ["W.lo"], [y1])
+        c.add(f"v_alignbit_b32 {{V.lo}}, {y1t}, {y0t}, {s2}", ["V.lo"], [y1, y0])
+        monty(c, "A.lo", "V.lo", "W.lo", "0", dst)
+    elif typ == "BH":
+        c.add(f"v_alignbit_b32 {{W.lo}}, {y1t}, {y0t}, {s2}", ["W.lo"], [y1, y0])
+        c.add(f"v_lshrrev_b32 {{W.hi}}, {s2}, {y1t}", ["W.hi"], [y1])
+        c.add(f"v_lshlrev_b32 {{V.lo}}, {s1}, {y0t}", ["V.lo"], [y0])
+        monty(c, "0", "V.lo", "W.lo", "W.hi", dst)
+    else:
+        raise ValueError(typ)
+
+
+def ct_p2_chain(tag: str, base: int, typ: str) -> Chain:
+    """x' = x + t, y' = x - t with t = y * 2^e (mulpow2)."""
+    c = Chain(tag, base)
+    x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
+    mulpow2(c, typ, y0, y1, "A")
+    ct_addsub(c, tag, x0, x1)
+    return c
+
+
+def gs_p2_chain(tag: str, base: int, typ: str) -> Chain:
+    """x, y canonical: x' = canon(x + y) (two halves out), y' = (x - y) * 2^e."""
+    c = Chain(tag, base)
+    x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
+    gs_diff(c, x0, x1, y0, y1)
+    mulpow2(c, typ, "V.lo", "V.hi", f"%Y{tag}")
+    gs_sum(c, tag, x0, x1, y0, y1)
+    return c
 
 
 def ct_chain(tag: str, base: int, wkind: str) -> Chain:
@@ -106,6 +175,12 @@ def ct_chain(tag: str, base: int, wkind: str) -> Chain:
     x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
     w0, w1 = f"%{wkind}w{tag}0", f"%{wkind}w{tag}1"
     montmul(c, y0, y1, w0, w1)
+    ct_addsub(c, tag, x0, x1)
+    return c
+
+
+def ct_addsub(c: Chain, tag: str, x0: str, x1: str):
+    """Outputs %X = x + A, %Y = x - A for canonical A, one EPS correction each."""
     c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {{A.lo}}", ["U.lo", "S1"], [x0, "A.lo"])
     c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {{A.lo}}", ["V.lo", "S2"], [x0, "A.lo"])
     c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {{A.hi}}, {{S1}}", ["U.hi", "S1"], [x1, "A.hi", "S1"])
@@ -115,7 +190,6 @@ def ct_chain(tag: str, base: int, wkind: str) -> Chain:
     c.add(f"v_mad_u64_u32 {{%X{tag}}}, {{D}}, {{T1}}, 1, {{U}}", [f"%X{tag}"], ["T1", "U"])
     c.add(f"v_mad_i64_i32 {{%Y{tag}}}, {{D}}, {{T2}}, {{%k55}}, {{V}}", [f"%Y{tag}"],
           ["T2", "%k55", "V"])
-    return c
 
 
 def gs_chain(tag: str, base: int, wkind: str) -> Chain:
@@ -124,10 +198,7 @@ def gs_chain(tag: str, base: int, wkind: str) -> Chain:
     x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
     w0, w1 = f"%{wkind}w{tag}0", f"%{wkind}w{tag}1"
     # d = x - y mod P, canonical, into T1/T2 (via pair V)
-    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {y0}", ["V.lo", "S2"], [x0, y0])
-    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {y1}, {{S2}}", ["V.hi", "S2"], [x1, y1, "S2"])
-    c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
-    c.add("v_mad_i64_i32 {V}, {D}, {T2}, {%k55}, {V}", ["V"], ["T2", "%k55", "V"])
+    gs_diff(c, x0, x1, y0, y1)
     c.add("v_mov_b32 {T1}, {V.lo}", ["T1"], ["V.lo"])
     c.add("v_mov_b32 {T2}, {V.hi}", ["T2"], ["V.hi"])
     # y' = montmul(d, w) straight into the output pair; montmul rewrites T1/T2 only
@@ -136,7 +207,21 @@ def gs_chain(tag: str, base: int, wkind: str) -> Chain:
     montmul(sub, "{T1}", "{T2}", w0, w1, dst=f"%Y{tag}")
     for text, defs, uses in sub.ins:
         c.add(text, defs, ["T1" if u == "{T1}" else "T2" if u == "{T2}" else u for u in uses])
-    # x' = canon(x + y): s = x + y (carry S1); q = s + EPS (carry S2); q if S1 | S2
+    gs_sum(c, tag, x0, x1, y0, y1)
+    return c
+
+
+def gs_diff(c: Chain, x0: str, x1: str, y0: str, y1: str):
+    """V <- x - y mod P, canonical for canonical x, y."""
+    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {y0}", ["V.lo", "S2"], [x0, y0])
+    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {y1}, {{S2}}", ["V.hi", "S2"], [x1, y1, "S2"])
+    c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
+    c.add("v_mad_i64_i32 {V}, {D}, {T2}, {%k55}, {V}", ["V"], ["T2", "%k55", "V"])
+
+
+def gs_sum(c: Chain, tag: str, x0: str, x1: str, y0: str, y1: str):
+    """Outputs %X<tag>lo/hi = canon(x + y): s = x + y (carry S1); q = s + EPS (carry S2);
+    q if S1 | S2 else s."""
     c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {y0}", ["U.lo", "S1"], [x0, y0])
     c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {y1}, {{S1}}", ["U.hi", "S1"], [x1, y1, "S1"])
     c.add("v_mad_u64_u32 {W}, {S2}, -1, 1, {U}", ["W", "S2"], ["U"])
@@ -145,7 +230,6 @@ def gs_chain(tag: str, base: int, wkind: str) -> Chain:
           ["U.lo", "W.lo", "S1"])
     c.add(f"v_cndmask_b32 {{%X{tag}hi}}, {{U.hi}}, {{W.hi}}, {{S1}}", [f"%X{tag}hi"],
           ["U.hi", "W.hi", "S1"])
-    return c
 
 
 def mac_chain(tag: str, base: int) -> Chain:
@@ -371,6 +455,27 @@ def main():
         parts.append(emit(f"gs2_{wkind}", [a, b], outputs, inputs,
                           f"Two GS butterflies (x, y) <- (x + y, (x - y)*w) on canonical "
                           f"inputs, twiddles in {'SGPRs' if wkind == 's' else 'VGPRs'}."))
+    # power-of-two twiddles: one macro per pair of multiplier types
+    for ta in ("A", "BL", "BH"):
+        for tb in ("A", "BL", "BH"):
+            inputs = []
+            for t in "ab":
+                inputs += [(f"x{t}0", "v", f"(u32)x{t}"), (f"x{t}1", "v", f"(u32)(x{t} >> 32)"),
+                           (f"y{t}0", "v", f"(u32)y{t}"), (f"y{t}1", "v", f"(u32)(y{t} >> 32)"),
+                           (f"s{t}1", "n", f"S{t.upper()}1"), (f"s{t}2", "n", f"S{t.upper()}2")]
+            inputs.append(("k55", "s", "0x55555555u"))
+            a, b = ct_p2_chain("a", TEMP_BASE, ta), ct_p2_chain("b", TEMP_BASE + 12, tb)
+            outputs = [("Xa", "=v", "Xa"), ("Ya", "=v", "Ya"), ("Xb", "=v", "Xb"),
+                       ("Yb", "=v", "Yb")]
+            parts.append(emit(f"ctp2_{ta}_{tb}", [a, b], outputs, inputs,
+                              f"Two CT butterflies with power-of-two twiddles ({ta}, {tb}): "
+                              "(x, y) <- (x + t, x - t), t = y * 2^e by shifts."))
+            a, b = gs_p2_chain("a", TEMP_BASE, ta), gs_p2_chain("b", TEMP_BASE + 12, tb)
+            outputs = [("Xalo", "=v", "Xalo"), ("Xahi", "=v", "Xahi"), ("Ya", "=v", "Ya"),
+                       ("Xblo", "=v", "Xblo"), ("Xbhi", "=v", "Xbhi"), ("Yb", "=v", "Yb")]
+            parts.append(emit(f"gsp2_{ta}_{tb}", [a, b], outputs, inputs,
+                              f"Two GS butterflies with power-of-two twiddles ({ta}, {tb}) on "
+                              "canonical inputs: (x, y) <- (x + y, (x - y) * 2^e)."))
     a, b = mac_chain("a", TEMP_BASE), mac_chain("b", TEMP_BASE + 12)
     outputs = [("Pa", "=v", "Pa"), ("Pb", "=v", "Pb")]
     inputs = [("x0", "v", "(u32)x"), ("x1", "v", "(u32)(x >> 32)")]

@@ -19,6 +19,8 @@
 // The reference delegates these transforms to concrete-ntt
 // (algebra/src/transformation/concrete.rs:51-61, 138-148).
 #pragma once
+#include <utility>
+
 #include "hb_field.hpp"
 
 namespace hb {
@@ -135,6 +137,42 @@ __device__ __forceinline__ void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb)
   HB_GL_MAC2_ASM;
   pa = Pa, pb = Pb;
 }
+
+// Butterfly pairs whose twiddles are +-2^e (hb_gl_asm.inc power-of-two macros). TA/TB:
+// multiplier type 0 (A), 1 (BL), 2 (BH) with shift immediates S*1, S*2; NA/NB: the
+// twiddle is the negation of what the macro multiplies by. For CT a negated twiddle
+// swaps the two outputs; for GS it swaps the inputs (the sum is symmetric, the
+// difference changes sign).
+#define HB_GL_P2_DISPATCH(KIND)                                              \
+  if constexpr (TA == 0 && TB == 0) { HB_GL_##KIND##P2_A_A_ASM; }            \
+  else if constexpr (TA == 0 && TB == 1) { HB_GL_##KIND##P2_A_BL_ASM; }      \
+  else if constexpr (TA == 0 && TB == 2) { HB_GL_##KIND##P2_A_BH_ASM; }      \
+  else if constexpr (TA == 1 && TB == 0) { HB_GL_##KIND##P2_BL_A_ASM; }      \
+  else if constexpr (TA == 1 && TB == 1) { HB_GL_##KIND##P2_BL_BL_ASM; }     \
+  else if constexpr (TA == 1 && TB == 2) { HB_GL_##KIND##P2_BL_BH_ASM; }     \
+  else if constexpr (TA == 2 && TB == 0) { HB_GL_##KIND##P2_BH_A_ASM; }      \
+  else if constexpr (TA == 2 && TB == 1) { HB_GL_##KIND##P2_BH_BL_ASM; }     \
+  else { HB_GL_##KIND##P2_BH_BH_ASM; }
+
+template <int TA, int SA1, int SA2, bool NA, int TB, int SB1, int SB2, bool NB>
+__device__ __forceinline__ void gl_ct2_pow2(u64& xa_, u64& ya_, u64& xb_, u64& yb_) {
+  const u64 xa = xa_, ya = ya_, xb = xb_, yb = yb_;
+  u64 Xa, Ya, Xb, Yb, dummy, s1a, s2a, s1b, s2b;
+  HB_GL_P2_DISPATCH(CT)
+  xa_ = NA ? Ya : Xa, ya_ = NA ? Xa : Ya;
+  xb_ = NB ? Yb : Xb, yb_ = NB ? Xb : Yb;
+}
+
+template <int TA, int SA1, int SA2, bool NA, int TB, int SB1, int SB2, bool NB>
+__device__ __forceinline__ void gl_gs2_pow2(u64& xa_, u64& ya_, u64& xb_, u64& yb_) {
+  const u64 xa = NA ? ya_ : xa_, ya = NA ? xa_ : ya_;
+  const u64 xb = NB ? yb_ : xb_, yb = NB ? xb_ : yb_;
+  u32 Xalo, Xahi, Xblo, Xbhi;
+  u64 Ya, Yb, dummy, s1a, s2a, s1b, s2b;
+  HB_GL_P2_DISPATCH(GS)
+  xa_ = ((u64)Xahi << 32) | Xalo, ya_ = Ya, xb_ = ((u64)Xbhi << 32) | Xblo, yb_ = Yb;
+}
+#undef HB_GL_P2_DISPATCH
 #else
 // Host compilation pass only parses device code; this body is never compiled for a GPU.
 __device__ inline void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb) {
@@ -179,13 +217,95 @@ struct Pass1Tw {
       w[k] = (W)ct_mulmod(ct_powmod(base, ct_bitrev(k, LOGN), p), r, p);
   }
 };
-// Index of the k-th butterfly's upper element in a stage whose partner distance is half.
-__host__ __device__ constexpr int bf_lo(int k, int half) { return (k / half) * 2 * half + k % half; }
-
 template <int E>
 struct Log2 {
   static constexpr int value = E == 16 ? 4 : E == 32 ? 5 : E == 8 ? 3 : -1;
 };
+
+// Index of the k-th butterfly's upper element in a stage whose partner distance is half.
+__host__ __device__ constexpr int bf_lo(int k, int half) { return (k / half) * 2 * half + k % half; }
+
+// Goldilocks pass-1 twiddles are powers of two: 2 has order 192 mod P, so the 2^LOGN-th
+// roots psi^bitrev(i) for i < E are +-2^e. Classification for the shift-based
+// multipliers of hb_gl_asm.inc (with 2^96 = -1):
+//   e in [1, 32):  type A, y * 2^e by shift and fold;
+//   e in [32, 96): y * 2^e = -(y * 2^(j - 64)), j = e - 32: types BL (j < 32) / BH (j > 32),
+//                  the Montgomery reduction of y * 2^j.
+// ok = 0 for anything else (the caller falls back to the general product).
+struct Pow2Tw {
+  int ok, neg, type, s1, s2;
+};
+template <int LOGN, bool INV>
+__host__ __device__ constexpr Pow2Tw pow2_twiddle(int i) {
+  const u64 p = Gl::P;
+  const u64 psi = ct_powmod(Gl::GENERATOR, (p - 1) >> (LOGN + 1), p);
+  const u64 base = INV ? ct_powmod(psi, p - 2, p) : psi;
+  const u64 w = ct_powmod(base, ct_bitrev(i, LOGN), p);
+  u64 v = 1;
+  int e = -1;
+  for (int k = 0; k < 192; k++) {
+    if (v == w) {
+      e = k;
+      break;
+    }
+    v = ct_mulmod(v, 2, p);
+  }
+  if (e < 0) return {0, 0, 0, 0, 0};
+  int neg = 0;
+  if (e >= 96) neg = 1, e -= 96;
+  if (e >= 1 && e < 32) return {1, neg, 0, e, 32 - e};
+  if (e >= 32 && e < 96) {
+    const int j = e - 32;
+    if (j >= 1 && j < 32) return {1, !neg, 1, j, 32 - j};
+    if (j >= 33 && j < 64) return {1, !neg, 2, j - 32, 64 - j};
+  }
+  return {0, 0, 0, 0, 0};
+}
+
+// Pass 1 at compile-time indices: butterfly pair P of stage S (twiddle indices known to
+// the compiler), Goldilocks twiddles as shift-based multipliers where possible.
+template <class F, int E, bool INV, int S, int P>
+__device__ __forceinline__ void pass1_pair(typename DF<F>::W (&x)[E]) {
+  constexpr int LOGE = Log2<E>::value;
+  constexpr int m = 1 << S, half = (E / 2) >> S;
+  constexpr int e0 = bf_lo(2 * P, half), e1 = bf_lo(2 * P + 1, half);
+  constexpr int ia = m + e0 / (2 * half), ib = m + e1 / (2 * half);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (F::IS_GOLDILOCKS) {
+    constexpr Pow2Tw a = pow2_twiddle<LOGE + 6, INV>(ia), b = pow2_twiddle<LOGE + 6, INV>(ib);
+    if constexpr (a.ok && b.ok) {
+      if constexpr (!INV)
+        gl_ct2_pow2<a.type, a.s1, a.s2, (bool)a.neg, b.type, b.s1, b.s2, (bool)b.neg>(
+            x[e0], x[e0 + half], x[e1], x[e1 + half]);
+      else
+        gl_gs2_pow2<a.type, a.s1, a.s2, (bool)a.neg, b.type, b.s1, b.s2, (bool)b.neg>(
+            x[e0], x[e0 + half], x[e1], x[e1 + half]);
+      return;
+    }
+  }
+#endif
+  constexpr Pass1Tw<F, LOGE + 6, INV> T{};
+  if constexpr (!INV)
+    Bfly<F>::ct2_u(x[e0], x[e0 + half], T.w[ia], x[e1], x[e1 + half], T.w[ib]);
+  else
+    Bfly<F>::gs2_u(x[e0], x[e0 + half], T.w[ia], x[e1], x[e1 + half], T.w[ib]);
+}
+
+template <class F, int E, bool INV, int S, int... Ps>
+__device__ __forceinline__ void pass1_stage(typename DF<F>::W (&x)[E],
+                                            std::integer_sequence<int, Ps...>) {
+  (pass1_pair<F, E, INV, S, Ps>(x), ...);
+}
+
+// Forward: stages S0 .. LOGE-1; inverse: LOGE-1 .. 0.
+template <class F, int E, bool INV, int S0, int... Is>
+__device__ __forceinline__ void pass1_all(typename DF<F>::W (&x)[E],
+                                          std::integer_sequence<int, Is...>) {
+  constexpr int LOGE = Log2<E>::value;
+  (pass1_stage<F, E, INV, INV ? LOGE - 1 - Is : S0 + Is>(
+       x, std::make_integer_sequence<int, E / 4>{}),
+   ...);
+}
 
 // Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
 // tw = psi_rev (psi^bitrev(k), constants in the field's constant form); pass 1 uses the
@@ -233,18 +353,8 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
     }
   }
   // pass 1
-#pragma unroll
-  for (int s = SMALL0 ? 1 : 0; s < LOGE; s++) {
-    const int m = 1 << s;
-    const int half = (E / 2) >> s;
-#pragma unroll
-    for (int p = 0; p < E / 4; p++) {
-      const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
-      constexpr Pass1Tw<F, LOGE + 6, false> T{};
-      Bfly<F>::ct2_u(x[e0], x[e0 + half], T.w[m + e0 / (2 * half)], x[e1], x[e1 + half],
-                     T.w[m + e1 / (2 * half)]);
-    }
-  }
+  constexpr int S0 = SMALL0 ? 1 : 0;
+  pass1_all<F, E, false, S0>(x, std::make_integer_sequence<int, LOGE - S0>{});
 #pragma unroll
   for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
   wave_sync();
@@ -386,18 +496,7 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
   for (int e = 0; e < E; e++) x[e] = lds[b1 + 68 * e];
   wave_sync();
   // pass 1
-#pragma unroll
-  for (int s = LOGE - 1; s >= 0; s--) {
-    const int m = 1 << s;
-    const int half = (E / 2) >> s;
-#pragma unroll
-    for (int p = 0; p < E / 4; p++) {
-      const int e0 = bf_lo(2 * p, half), e1 = bf_lo(2 * p + 1, half);
-      constexpr Pass1Tw<F, LOGE + 6, true> T{};
-      Bfly<F>::gs2_u(x[e0], x[e0 + half], T.w[m + e0 / (2 * half)], x[e1], x[e1 + half],
-                     T.w[m + e1 / (2 * half)]);
-    }
-  }
+  pass1_all<F, E, true, 0>(x, std::make_integer_sequence<int, LOGE>{});
   if constexpr (!F::IS_GOLDILOCKS) {  // the Goldilocks GS pair already returns canonical words
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = DF<F>::canon(x[e]);

@@ -36,6 +36,8 @@ def parse_macros():
 
 
 def c_eval(expr, env):
+    if expr in env:
+        return env[expr]
     e = expr.replace("0x55555555u", "0x55555555")
     e = re.sub(r"\(u32\)\(([^()]*)\)", r"u32(\1)", e)
     e = re.sub(r"\(u32\)(\w+)", r"u32(\1)", e)
@@ -113,6 +115,13 @@ class Lane:
                 self.op[int(a[1][1:])] = 1 if r < 0 else 0
             elif mn == "v_cndmask_b32":
                 self.wr32(a[0], self.rd32(a[2]) if self.bit(a[3]) else self.rd32(a[1]))
+            elif mn == "v_lshlrev_b32":
+                self.wr32(a[0], self.rd32(a[2]) << (self.rd32(a[1]) & 31))
+            elif mn == "v_lshrrev_b32":
+                self.wr32(a[0], self.rd32(a[2]) >> (self.rd32(a[1]) & 31))
+            elif mn == "v_alignbit_b32":
+                v = (self.rd32(a[1]) << 32) | self.rd32(a[2])
+                self.wr32(a[0], v >> (self.rd32(a[3]) & 31))
             elif mn == "s_or_b64":
                 self.op[int(a[0][1:])] = self.bit(a[1]) | self.bit(a[2])
             else:
@@ -188,3 +197,38 @@ def test_mac_pair():
         out = run_macro(mac, env)
         assert out["Pa"] % P == (env["pa"] + mont(env["x"], env["ka"])) % P
         assert out["Pb"] % P == (env["pb"] + mont(env["x"], env["kb"])) % P
+
+
+def _pow2_cases(typ):
+    """(s1, s2, exponent e of the multiplier 2^e mod P) for each shift of a type."""
+    if typ == "A":
+        return [(k, 32 - k, k) for k in range(1, 32)]
+    if typ == "BL":
+        return [(j, 32 - j, j - 64) for j in range(1, 32)]
+    return [(j - 32, 64 - j, j - 64) for j in range(33, 64)]
+
+
+@pytest.mark.parametrize("kind", ["CT", "GS"])
+@pytest.mark.parametrize("ta", ["A", "BL", "BH"])
+@pytest.mark.parametrize("tb", ["A", "BL", "BH"])
+def test_pow2_pairs(kind, ta, tb):
+    mac = parse_macros()[f"HB_GL_{kind}P2_{ta}_{tb}_ASM"]
+    rng = random.Random(hash((kind, ta, tb)) & 0xFFFF)
+    ca, cb = _pow2_cases(ta), _pow2_cases(tb)
+    canon = kind == "GS"
+    vals = samples(rng, 60, canonical=canon)
+    for i in range(max(len(ca), len(cb)) * 3):
+        sa, sb = ca[i % len(ca)], cb[(7 * i) % len(cb)]
+        env = dict(xa=vals[i % len(vals)], ya=vals[(3 * i + 1) % len(vals)],
+                   xb=vals[(5 * i + 2) % len(vals)], yb=vals[(11 * i + 3) % len(vals)],
+                   SA1=sa[0], SA2=sa[1], SB1=sb[0], SB2=sb[1])
+        out = run_macro(mac, env)
+        for t, (_, _, e) in (("a", sa), ("b", sb)):
+            x, y = env["x" + t], env["y" + t]
+            m = pow(2, e % 192, P)
+            if kind == "CT":
+                assert out["X" + t] % P == (x + y * m) % P, (t, e)
+                assert out["Y" + t] % P == (x - y * m) % P, (t, e)
+            else:
+                assert (out[f"X{t}lo"] | (out[f"X{t}hi"] << 32)) == (x + y) % P
+                assert out["Y" + t] == (x - y) * m % P, (t, e)
