@@ -205,9 +205,11 @@ def main() -> int:
 
     tune = None
     if not args.no_autotune:
-        tune = ev.autotune(probe_gates=min(G, 1024))
-        log(f"autotune: {tune['gates_per_workgroup']} gates per workgroup "
-            f"(probe ms {', '.join(f'{k}:{v:.1f}' for k, v in tune['ms'].items())})")
+        tune = ev.autotune(probe_gates=G)
+        log(f"autotune: {tune['gates_per_workgroup']} gates per workgroup, "
+            f"{tune['waves_per_simd']} waves per SIMD (probe ms "
+            f"{', '.join(f'{k}:{v:.1f}' for k, v in tune['ms'].items())})")
+        tune["ms"] = {f"{k[0]}x{k[1]}": round(v, 3) for k, v in tune["ms"].items()}
 
     graph = None
     if not args.no_graph:

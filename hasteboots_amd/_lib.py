@@ -50,8 +50,11 @@ class HbServerStats(C.Structure):
 
 
 class HbTuneResult(C.Structure):
-    _fields_ = [("gates_per_workgroup", C.c_int32), ("pad_", C.c_int32),
-                ("probe_gates", C.c_uint64), ("ms", C.c_double * 3)]
+    _fields_ = [("gates_per_workgroup", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("probe_gates", C.c_uint64), ("ms", C.c_double * 5)]
+
+
+TUNE_VARIANTS = [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2)]
 
 
 class HbError(RuntimeError):
@@ -86,8 +89,9 @@ _PROTOS = {
     "hb_batch_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "hb_batch_graph_destroy": (C.c_int, [C.c_void_p]),
     "hb_autotune": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(HbTuneResult)]),
-    "hb_tuned_config": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-    "hb_set_tuned_config": (C.c_int, [C.c_int, C.c_int]),
+    "hb_tuned_config": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
+    "hb_set_tuned_config": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "hb_server_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.POINTER(C.c_void_p)]),
     "hb_server_destroy": (C.c_int, [C.c_void_p]),
     "hb_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(HbCircuit), u64p,

@@ -56,19 +56,38 @@ uint64_t to_device_word(const ParamSet& p, uint64_t v, uint64_t scale) {
 
 namespace {
 
-// Autotune cache: text lines "<set> <gates per workgroup>", at the path named by the
-// HB_AUTOTUNE_CACHE environment variable (no file, no cache).
+// Autotune cache: text lines "<set> <gates per workgroup> <waves per SIMD>", at the path
+// named by the HB_AUTOTUNE_CACHE environment variable (no file, no cache).
 const char* tune_cache_path() { return std::getenv("HB_AUTOTUNE_CACHE"); }
+
+// The batched-kernel variants, in the order of hb_tune_result_t::ms.
+const int kVariants[5][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 2}};
+
+// Variant used for a batch of `count` gates of parameter set `set`.
+void batch_variant(const Engine& E, int set, size_t count, int* gpw, int* wps) {
+  if (count <= E.simd_slots) {
+    *gpw = 1, *wps = 1;
+  } else {
+    *gpw = E.gpw[set], *wps = E.wps[set];
+  }
+}
+
+bool valid_variant(int gpw, int wps) {
+  for (const auto& v : kVariants)
+    if (v[0] == gpw && v[1] == wps) return true;
+  return false;
+}
 
 void load_tune_cache(Engine& E) {
   const char* path = tune_cache_path();
   if (!path) return;
   FILE* f = std::fopen(path, "r");
   if (!f) return;
-  int set, gpw;
-  while (std::fscanf(f, "%d %d", &set, &gpw) == 2) {
-    if (set >= 0 && set < 3 && (gpw == 1 || gpw == 2 || gpw == 4)) {
+  int set, gpw, wps;
+  while (std::fscanf(f, "%d %d %d", &set, &gpw, &wps) == 3) {
+    if (set >= 0 && set < 3 && valid_variant(gpw, wps)) {
       E.gpw[set] = gpw;
+      E.wps[set] = wps;
       E.tuned[set] = true;
     }
   }
@@ -81,7 +100,7 @@ void save_tune_cache(const Engine& E) {
   FILE* f = std::fopen(path, "w");
   if (!f) return;
   for (int set = 0; set < 3; set++)
-    if (E.tuned[set]) std::fprintf(f, "%d %d\n", set, E.gpw[set]);
+    if (E.tuned[set]) std::fprintf(f, "%d %d %d\n", set, E.gpw[set], E.wps[set]);
   std::fclose(f);
 }
 
@@ -129,6 +148,11 @@ int hb::ensure_engine(int device) {
   }
   E.tables.ready = true;
   E.device = device;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
+  }
   load_tune_cache(E);
   return HB_OK;
 }
@@ -260,7 +284,9 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   if (rc) return rc;
   BootArgs a = make_args(ek, gate, count, c0, c1, c2, lut_dev, out, ek->acc_dev);
   HIP_TRY(hipEventRecord(E.ev0, s));
-  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, E.gpw[p.set], s));
+  int gpw, wps;
+  batch_variant(E, p.set, count, &gpw, &wps);
+  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
   return HB_OK;
 }
@@ -596,7 +622,7 @@ int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const
   BootArgs a = make_args(ek, gate, count, d0, c1s ? d1 : nullptr, c2s ? d2 : nullptr,
                          lut ? (const uint64_t*)ek->lut_dev : nullptr, dout, ek->acc_dev);
   a.trace = tr;
-  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, 1, E.stream) != hipSuccess ||
+  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, 1, 1, E.stream) != hipSuccess ||
       hipEventRecord(E.ev1, E.stream) != hipSuccess)
     return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
   std::vector<uint8_t> host(words * wb);
@@ -807,7 +833,9 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    hipError_t le = launch_bootstrap(a, ek->p.is_goldilocks, E.gpw[ek->p.set], cs);
+    int gpw, wps;
+    batch_variant(E, ek->p.set, count, &gpw, &wps);
+    hipError_t le = launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
     e = hipStreamEndCapture(cs, &g->graph);
     if (le != hipSuccess) e = le;
   }
@@ -873,13 +901,13 @@ int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out) {
   uint64_t* c1 = c0 + probe_gates * n1;
   uint64_t* o = c1 + probe_gates * n1;
   BootArgs a = make_args(ek, GATE_NAND, probe_gates, c0, c1, nullptr, nullptr, o, ek->acc_dev);
-  static const int kGpw[3] = {1, 2, 4};
   double best = 1e300;
-  int best_gpw = 1;
-  for (int v = 0; v < 3; v++) {
-    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, kGpw[v], E.stream));  // warm-up
+  int best_v = 0;
+  for (int v = 0; v < 5; v++) {
+    const int gpw = kVariants[v][0], wps = kVariants[v][1];
+    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, E.stream));  // warm-up
     HIP_TRY(hipEventRecord(E.ev0, E.stream));
-    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, kGpw[v], E.stream));
+    HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, E.stream));
     HIP_TRY(hipEventRecord(E.ev1, E.stream));
     HIP_TRY(hipEventSynchronize(E.ev1));
     float ms = 0;
@@ -887,37 +915,42 @@ int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out) {
     out->ms[v] = ms;
     if (ms < best) {
       best = ms;
-      best_gpw = kGpw[v];
+      best_v = v;
     }
   }
-  E.gpw[ek->p.set] = best_gpw;
+  E.gpw[ek->p.set] = kVariants[best_v][0];
+  E.wps[ek->p.set] = kVariants[best_v][1];
   E.tuned[ek->p.set] = true;
   save_tune_cache(E);
-  out->gates_per_workgroup = best_gpw;
+  out->gates_per_workgroup = kVariants[best_v][0];
+  out->waves_per_simd = kVariants[best_v][1];
   out->probe_gates = probe_gates;
   return HB_OK;
 }
 
-int hb_tuned_config(int set, int* gates_per_workgroup, int* tuned) {
-  if (set < 0 || set > 2 || !gates_per_workgroup) return fail(HB_ERR_ARG, "bad argument");
+int hb_tuned_config(int set, int* gates_per_workgroup, int* waves_per_simd, int* tuned) {
+  if (set < 0 || set > 2 || !gates_per_workgroup || !waves_per_simd)
+    return fail(HB_ERR_ARG, "bad argument");
   int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
   if (rc) return rc;
   *gates_per_workgroup = engine().gpw[set];
+  *waves_per_simd = engine().wps[set];
   if (tuned) *tuned = engine().tuned[set] ? 1 : 0;
   return HB_OK;
 }
 
-int hb_set_tuned_config(int set, int gates_per_workgroup) {
+int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd) {
   if (set < 0 || set > 2) return fail(HB_ERR_ARG, "bad parameter set %d", set);
-  if (gates_per_workgroup != 1 && gates_per_workgroup != 2 && gates_per_workgroup != 4)
-    return fail(HB_ERR_ARG, "gates per workgroup must be 1, 2 or 4");
+  if (!valid_variant(gates_per_workgroup, waves_per_simd))
+    return fail(HB_ERR_ARG, "no kernel variant with %d gates per workgroup, %d waves per SIMD",
+                gates_per_workgroup, waves_per_simd);
   int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
   if (rc) return rc;
   engine().gpw[set] = gates_per_workgroup;
+  engine().wps[set] = waves_per_simd;
   engine().tuned[set] = true;
   return HB_OK;
 }
-
 int hb_modmul_peak(uint32_t waves, uint32_t iters, double* modmul_per_s, double* ms_out) {
   if (!modmul_per_s || waves == 0 || iters == 0 || waves > (1u << 20))
     return fail(HB_ERR_ARG, "bad modmul peak arguments");

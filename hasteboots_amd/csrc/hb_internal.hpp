@@ -45,10 +45,16 @@ struct Engine {
   size_t mixed_parts_cap = 0;
   uint64_t* mixed_io = nullptr;
   size_t mixed_io_words = 0;
-  // Autotuned gates (wavefronts) per workgroup of the batched kernel, per parameter
-  // set (HB_SET_*); 1 until hb_autotune runs or a cache file supplies a value.
-  int gpw[3] = {1, 1, 1};
+  // Batched-kernel variant per parameter set (HB_SET_*): gates (wavefronts) per
+  // workgroup and waves per SIMD. The default is the variant measured fastest on MI355X
+  // for Goldilocks; hb_autotune or the cache file replaces it.
+  int gpw[3] = {4, 4, 4};
+  int wps[3] = {2, 2, 2};
   bool tuned[3] = {false, false, false};
+  // Batches of at most this many gates (one wave per SIMD across the device) run the
+  // 1-gate-per-workgroup, 1-wave-per-SIMD variant: spreading them over every SIMD beats
+  // doubling up waves on half of them.
+  uint32_t simd_slots = 1024;
 };
 
 Engine& engine();

@@ -372,8 +372,12 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
   __syncthreads();
 }
 
-template <class F, int GPW>
-__global__ __launch_bounds__(64 * GPW) void bootstrap_kernel(BootArgs A) {
+// WPS: waves per SIMD the register allocation targets (1: up to 512 VGPR+AGPR per lane;
+// 2: at most 256, the excess spilled, so that two waves per SIMD hide each other's
+// LDS and memory latency).
+template <class F, int GPW, int WPS>
+__global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
+void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[GPW][1024 + 64];
   __shared__ unsigned short ms[GPW][1024];
@@ -526,25 +530,27 @@ __global__ __launch_bounds__(64) void modmul_peak_kernel(u64* __restrict__ out, 
 // Launchers
 // ----------------------------------------------------------------------------
 template <class F>
-static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, hipStream_t s) {
-  switch (gpw) {
-    case 1:
-      hipLaunchKernelGGL((bootstrap_kernel<F, 1>), dim3(a.count), dim3(64), 0, s, a);
-      break;
-    case 2:
-      hipLaunchKernelGGL((bootstrap_kernel<F, 2>), dim3((a.count + 1) / 2), dim3(128), 0, s, a);
-      break;
-    case 4:
-      hipLaunchKernelGGL((bootstrap_kernel<F, 4>), dim3((a.count + 3) / 4), dim3(256), 0, s, a);
-      break;
-    default:
-      return hipErrorInvalidValue;
-  }
+static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStream_t s) {
+  const dim3 g1(a.count), g2((a.count + 1) / 2), g4((a.count + 3) / 4);
+  if (gpw == 1 && wps == 1)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 1, 1>), g1, dim3(64), 0, s, a);
+  else if (gpw == 2 && wps == 1)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 1>), g2, dim3(128), 0, s, a);
+  else if (gpw == 4 && wps == 1)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 1>), g4, dim3(256), 0, s, a);
+  else if (gpw == 2 && wps == 2)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2>), g2, dim3(128), 0, s, a);
+  else if (gpw == 4 && wps == 2)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2>), g4, dim3(256), 0, s, a);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, hipStream_t s) {
-  return is_goldilocks ? launch_bootstrap_f<Gl>(a, gpw, s) : launch_bootstrap_f<Bb>(a, gpw, s);
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
+                            hipStream_t s) {
+  return is_goldilocks ? launch_bootstrap_f<Gl>(a, gpw, wps, s)
+                       : launch_bootstrap_f<Bb>(a, gpw, wps, s);
 }
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {

@@ -42,8 +42,10 @@ struct BootArgs {
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
 };
 
-// gpw: gates (wavefronts) per workgroup, 1, 2 or 4.
-hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, hipStream_t s);
+// Batched-kernel variants: gpw gates (wavefronts) per workgroup and wps waves per SIMD
+// targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2)}.
+hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
+                            hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);

@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import HbTuneResult, HbParams, check, lib, ptr
+from ._lib import TUNE_VARIANTS, HbParams, HbTuneResult, check, lib, ptr
 
 SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA = 0, 1, 2
 GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6}
@@ -279,12 +279,13 @@ class Evaluator:
         return self.gate_batch("nand", c0s, c1s)
 
     def autotune(self, probe_gates: int = 1024) -> dict:
-        """Times the 1/2/4 gates-per-workgroup variants of the batched kernel for this
-        parameter set and keeps the fastest (hb_autotune)."""
+        """Times the batched kernel's variants for this parameter set and keeps the
+        fastest (hb_autotune). ms is keyed by (gates per workgroup, waves per SIMD)."""
         r = HbTuneResult()
         check(lib().hb_autotune(self._h, probe_gates, C.byref(r)))
-        return {"gates_per_workgroup": r.gates_per_workgroup, "probe_gates": r.probe_gates,
-                "ms": {1: r.ms[0], 2: r.ms[1], 4: r.ms[2]}}
+        return {"gates_per_workgroup": r.gates_per_workgroup,
+                "waves_per_simd": r.waves_per_simd, "probe_gates": r.probe_gates,
+                "ms": {v: r.ms[i] for i, v in enumerate(TUNE_VARIANTS)}}
 
     def batch_graph(self, gate: str, count: int):
         """A captured (hipGraph) batch launch over device buffers owned by the returned
@@ -395,15 +396,15 @@ class BatchGraph:
             pass
 
 
-def tuned_config(set_id: int) -> tuple[int, bool]:
-    """(gates per workgroup, whether it was tuned) for a parameter set."""
-    g, t = C.c_int(), C.c_int()
-    check(lib().hb_tuned_config(set_id, C.byref(g), C.byref(t)))
-    return g.value, bool(t.value)
+def tuned_config(set_id: int) -> tuple[int, int, bool]:
+    """(gates per workgroup, waves per SIMD, whether it was tuned) for a parameter set."""
+    g, w, t = C.c_int(), C.c_int(), C.c_int()
+    check(lib().hb_tuned_config(set_id, C.byref(g), C.byref(w), C.byref(t)))
+    return g.value, w.value, bool(t.value)
 
 
-def set_tuned_config(set_id: int, gates_per_workgroup: int) -> None:
-    check(lib().hb_set_tuned_config(set_id, gates_per_workgroup))
+def set_tuned_config(set_id: int, gates_per_workgroup: int, waves_per_simd: int) -> None:
+    check(lib().hb_set_tuned_config(set_id, gates_per_workgroup, waves_per_simd))
 
 
 def mixed_batch(parts) -> list[np.ndarray]:

@@ -136,22 +136,28 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
 int hb_batch_graph_launch(hb_batch_graph_t* graph, void* stream);
 int hb_batch_graph_destroy(hb_batch_graph_t* graph);
 
-/* Autotuner for the batched kernel: times the variants with 1, 2 and 4 gates
- * (wavefronts) per workgroup on `probe_gates` gates with this key's parameter set, keeps
- * the fastest for that set (used by every later batch launch of the set) and, when the
- * environment variable HB_AUTOTUNE_CACHE names a file, stores the choice there; a new
- * process reads it back. No reference counterpart. */
+/* Autotuner for the batched kernel: times its variants on `probe_gates` gates with this
+ * key's parameter set and keeps the fastest for that set (used by every later batch
+ * launch of the set); when the environment variable HB_AUTOTUNE_CACHE names a file the
+ * choice is stored there and a new process reads it back. Variants, in the order of
+ * ms[]: (gates per workgroup, waves per SIMD) = (1,1), (2,1), (4,1), (2,2), (4,2).
+ * Gates per workgroup share one LDS copy of the twiddle tables; 2 waves per SIMD caps
+ * registers at 256 per lane so two waves hide each other's latency. The tuned variant
+ * applies to batches larger than one wave per SIMD of the device (4 x compute units);
+ * smaller batches always run (1,1). Probe with at least twice that many gates. No
+ * reference counterpart. */
 typedef struct {
   int gates_per_workgroup;
-  int pad_;
+  int waves_per_simd;
   uint64_t probe_gates;
-  double ms[3]; /* kernel time of the 1, 2 and 4 gates-per-workgroup variants */
+  double ms[5]; /* kernel time per variant */
 } hb_tune_result_t;
 int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out);
-/* Current choice for a parameter set; *tuned = 1 if it came from hb_autotune or the cache. */
-int hb_tuned_config(int set, int* gates_per_workgroup, int* tuned);
-/* Pins the variant for a parameter set (1, 2 or 4 gates per workgroup). */
-int hb_set_tuned_config(int set, int gates_per_workgroup);
+/* Current variant for a parameter set; *tuned = 1 if it came from hb_autotune, the cache
+ * or hb_set_tuned_config (otherwise it is the built-in default). */
+int hb_tuned_config(int set, int* gates_per_workgroup, int* waves_per_simd, int* tuned);
+/* Pins the variant for a parameter set. */
+int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
 
 /* Mixed-parameter batch: several parts, each with its own evaluation key (any of the
  * three parameter sets) and gate kind, bootstrapped in ONE kernel launch. No reference
