@@ -30,6 +30,7 @@ SET_NAMES = {0: "BABYBEAR_BINARY_128_BITS", 1: "GOLDILOCKS_BINARY_128_BITS",
              2: "ZAMA_BINARY_128_BITS"}
 METRIC = "NAND bootstraps/sec, batched vs plain on 1 MI355X; p50/p99 gate latency"
 HBM_PEAK = 8.0e12
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second
 
 
 def log(*a):
@@ -111,6 +112,16 @@ def microbench(L, check, hb, reps: int = 3) -> dict:
     L.hb_dev_free(rgsw)
     L.hb_dev_free(rl)
     return out
+
+
+def profiled_valu(set_name: str, gates: int):
+    """VALU instructions per gate (wave64 instructions) counted by rocprofv3 SQ_INSTS_VALU
+    for this workload, recorded in profiles/; (None, None) if absent."""
+    p = ROOT / "profiles" / f"bootstrap_kernel_{set_name}_{gates}_valu.md"
+    if not p.exists():
+        return None, None
+    m = re.search(r"valu_instructions_per_gate:\s*(\d+)", p.read_text())
+    return (int(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
 def cpu_model() -> str:
@@ -314,6 +325,8 @@ def main() -> int:
     achieved = traffic_alg / (kms * 1e-3)
     mm = modmuls_per_gate(P)
     traffic, traffic_src = profiled_traffic(args.set, G)
+    valu_per_gate, valu_src = profiled_valu(args.set, G)
+    valu_rate = valu_per_gate * G / (kms * 1e-3) if valu_per_gate else None
 
     cpu = None
     if not args.no_cpu:
@@ -369,13 +382,19 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK, 5), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": traffic_alg},
-        "valu": {"modmul_per_gate": mm,
+        "valu": {"bound": "valu_issue",
+                 "achieved": round(valu_rate, 1) if valu_rate else None,
+                 "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
+                 "frac": round(valu_rate / VALU_PEAK, 4) if valu_rate else None,
+                 "instructions_per_gate": valu_per_gate, "instructions_source": valu_src,
+                 "peak_basis": "256 CUs x 4 SIMDs x one wave64 VALU instruction per 4 clocks "
+                               "at 2.4 GHz",
+                 "modmul_per_gate": mm,
                  "achieved_modmul_per_s": round(mm * G / (kms * 1e-3), 1),
-                 "peak_modmul_per_s": round(micro["modmul_peak_per_s"], 1) if micro else None,
-                 "frac": round(mm * G / (kms * 1e-3) / micro["modmul_peak_per_s"], 4)
-                 if micro else None,
-                 "peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
-                                "8192 wavefronts"},
+                 "mac_peak_modmul_per_s": round(micro["modmul_peak_per_s"], 1) if micro
+                 else None,
+                 "mac_peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
+                                    "8192 wavefronts"},
         "microbench": micro,
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
                     "oracle_gates": args.check},
