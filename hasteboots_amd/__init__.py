@@ -19,7 +19,9 @@ from .tfhe import (  # noqa: F401
     SecretKeyPack,
     device_count,
     last_kernel_ms,
+    latency_mode,
     mixed_batch,
+    set_latency_mode,
     set_tuned_config,
     tuned_config,
 )

@@ -114,6 +114,8 @@ _PROTOS = {
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),
     "hb_external_product_dev": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "hb_set_latency_mode": (C.c_int, [C.c_uint32]),
+    "hb_latency_mode": (C.c_int, [C.POINTER(C.c_uint32)]),
     "hb_modmul_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
     "hb_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),

@@ -63,9 +63,12 @@ const char* tune_cache_path() { return std::getenv("HB_AUTOTUNE_CACHE"); }
 // The batched-kernel variants, in the order of hb_tune_result_t::ms.
 const int kVariants[5][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 2}};
 
-// Variant used for a batch of `count` gates of parameter set `set`.
+// Variant used for a batch of `count` gates of parameter set `set`; gpw = 0 selects the
+// latency-mode kernel (4 wavefronts per gate).
 void batch_variant(const Engine& E, int set, size_t count, int* gpw, int* wps) {
-  if (count <= E.simd_slots) {
+  if (count <= E.split_max) {
+    *gpw = 0, *wps = 1;
+  } else if (count <= E.simd_slots) {
     *gpw = 1, *wps = 1;
   } else {
     *gpw = E.gpw[set], *wps = E.wps[set];
@@ -152,6 +155,7 @@ int hb::ensure_engine(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
+    E.split_max = E.simd_slots / 4;
   }
   load_tune_cache(E);
   return HB_OK;
@@ -286,7 +290,10 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   HIP_TRY(hipEventRecord(E.ev0, s));
   int gpw, wps;
   batch_variant(E, p.set, count, &gpw, &wps);
-  HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
+  if (gpw == 0)
+    HIP_TRY(launch_bootstrap_split(a, p.is_goldilocks, s));
+  else
+    HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
   return HB_OK;
 }
@@ -835,7 +842,8 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   if (e == hipSuccess) {
     int gpw, wps;
     batch_variant(E, ek->p.set, count, &gpw, &wps);
-    hipError_t le = launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
+    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks, cs)
+                             : launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
     e = hipStreamEndCapture(cs, &g->graph);
     if (le != hipSuccess) e = le;
   }
@@ -972,5 +980,20 @@ int hb_modmul_peak(uint32_t waves, uint32_t iters, double* modmul_per_s, double*
   // per lane and iteration: 4 gl_mac2 = 8 products
   *modmul_per_s = (double)waves * 64.0 * iters * 8.0 / (ms * 1e-3);
   if (ms_out) *ms_out = ms;
+  return HB_OK;
+}
+
+int hb_set_latency_mode(uint32_t max_gates) {
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  engine().split_max = max_gates;
+  return HB_OK;
+}
+
+int hb_latency_mode(uint32_t* max_gates) {
+  if (!max_gates) return fail(HB_ERR_ARG, "null argument");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  *max_gates = engine().split_max;
   return HB_OK;
 }

@@ -55,6 +55,9 @@ struct Engine {
   // 1-gate-per-workgroup, 1-wave-per-SIMD variant: spreading them over every SIMD beats
   // doubling up waves on half of them.
   uint32_t simd_slots = 1024;
+  // Batches of at most this many untraced gates run in latency mode (4 wavefronts per
+  // gate); by default a quarter of the SIMDs, so one such batch still fills the device.
+  uint32_t split_max = 256;
 };
 
 Engine& engine();

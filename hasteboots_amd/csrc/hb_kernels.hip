@@ -189,49 +189,38 @@ __device__ __forceinline__ void gate_sync() {
 //   → acc.b[0] += Q/8 → KeySwitchingRLWEKey::key_switch_for_rlwe
 //   (fhe_core/src/key_switch/rlwe.rs:115-137) → extract_partial_lwe_locally.
 // ----------------------------------------------------------------------------
+// Step 1 of a gate, by threads tid = 0..nthreads-1 of its workgroup (no barrier inside):
+// gate combination and modulus switch of all n+1 coefficients into ms. Inputs are dense
+// arrays (gate g at row g) or, for circuits, wire indices into one device-resident wire
+// buffer: idx[3g..3g+2] = (input 0, input 1, output).
 template <class F>
-__device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
-                                               unsigned short* ms,
-                                               const typename DF<F>::W* twl,
-                                               const typename DF<F>::W* itwl) {
+__device__ __forceinline__ void gate_modswitch(const BootArgs& A, int g, int tid, int nthreads,
+                                               unsigned short* ms) {
   using W = typename DF<F>::W;
-  constexpr int E = 16;
-  constexpr int N = 64 * E;
-  const int lane = threadIdx.x & 63;
-  const u32 n = A.n, n1 = A.n + 1;
-  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
-  const W* __restrict__ itw = static_cast<const W*>(A.tw_inv);
-  const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
-  const W* __restrict__ ksk = static_cast<const W*>(A.ksk);
-  // Untraced: one accumulator per gate, updated in place. Traced (the vfhe path): an
-  // (n+1)-slot history per gate; round i reads slot i and writes slot i+1, so the
-  // history is the core trace (the acc entering every round, then the final acc).
-  const bool traced = A.trace != nullptr;
-  W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N
-                   : static_cast<W*>(A.acc) + (size_t)g * 2 * N;
-  W* accB = accA + N;
-
-  // 1. gate combination + modulus switch of all n+1 coefficients.
-  // Either dense arrays (gate g at row g) or, for circuits, wire indices into one
-  // device-resident wire buffer: idx[3g..3g+2] = (input 0, input 1, output).
+  constexpr int N = 1024;
+  const u32 n1 = A.n + 1;
   const u32* wi = A.idx ? A.idx + 3 * (size_t)g : nullptr;
   const u64* c0 = A.c0 + (size_t)(wi ? wi[0] : g) * n1;
   const u64* c1 = wi ? A.c0 + (size_t)wi[1] * n1 : (A.c1 ? A.c1 + (size_t)g * n1 : nullptr);
   const u64* c2 = (!wi && A.c2) ? A.c2 + (size_t)g * n1 : nullptr;
-  for (u32 k = lane; k < n1; k += 64) {
+  for (u32 k = tid; k < n1; k += nthreads) {
     W a = (W)c0[k];
     W b = c1 ? (W)c1[k] : (W)0;
     W c = c2 ? (W)c2[k] : (W)0;
     W v = combine<F>(A.gate, a, b, c);
     ms[k] = (unsigned short)mod_switch((u64)v, A.qf, 2 * N);
   }
-  wave_sync();
+}
 
-  // 2. ACC = (0, LUT * X^-b)  (binary.rs:40-64)
-  const u32 bms = ms[n];
-#pragma unroll
-  for (int e = 0; e < E; e++) {
-    const int k = lane + 64 * e;
+// Step 2 (after step 1 is visible): ACC = (0, LUT * X^-b) (binary.rs:40-64).
+template <class F>
+__device__ __forceinline__ void gate_lut_init(const BootArgs& A, int tid, int nthreads,
+                                              const unsigned short* ms, typename DF<F>::W* accA,
+                                              typename DF<F>::W* accB) {
+  using W = typename DF<F>::W;
+  constexpr int N = 1024;
+  const u32 bms = ms[A.n];
+  for (int k = tid; k < N; k += nthreads) {
     W val;
     if (bms == 0) {
       val = lut_at<F>(A, k);
@@ -253,78 +242,51 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
     accA[k] = 0;
     accB[k] = val;
   }
-  gate_sync();
+}
 
-  // 3. blind rotation: for a_i != 0, ACC += ExtProd((X^a_i - 1) ACC, RGSW(s_i)).
-  const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
-  const size_t step = traced ? 2 * N : 0;
-  for (u32 i = 0; i < n; i++, accA += step, accB += step) {
-    const int r = ms[i];
-    W* outA = accA + step;
-    W* outB = accB + step;
-    if (r == 0) {  // binary.rs:70 skips the round; the vfhe path adds a zero product
-      if (traced) {
+// RLWE::mul_monic_monomial_sub_one_inplace (lattice/src/rlwe.rs:437-487) for one component:
+// dst[e] = ±acc[k - r mod N] - acc[k] at k = lane + 64e, the ± being the raw-word negation
+// (untraced) or the exact negation of the canonical value (traced: the NTT product
+// ACC * X^r of binary.rs:158-181).
+template <class F, int E>
+__device__ __forceinline__ void rotate_sub(const typename DF<F>::W* acc, int r, bool traced,
+                                           int lane, typename DF<F>::W (&dst)[E]) {
+  using W = typename DF<F>::W;
+  constexpr int N = 64 * E;
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          const int k = lane + 64 * e;
-          outA[k] = accA[k];
-          outB[k] = accB[k];
-        }
-        gate_sync();
-      }
-      continue;
+  for (int e = 0; e < E; e++) {
+    const int k = lane + 64 * e;
+    bool neg;
+    int src;
+    if (r <= N) {
+      neg = k < r;
+      src = neg ? k - r + N : k - r;
+    } else {
+      const int rr = r - N;
+      neg = k >= rr;
+      src = neg ? k - rr : k - rr + N;
     }
-    const W* key = bsk + (size_t)i * rgsw_words;
-    W pa[E], pb[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
-#pragma unroll 1
-    for (int comp = 0; comp < 2; comp++) {
-      const W* acc = comp ? accB : accA;
-      // RLWE::mul_monic_monomial_sub_one_inplace (lattice/src/rlwe.rs:437-487):
-      // dst[k] = ±acc[k - r mod N] - acc[k], the ± being the raw-word negation.
-      W dst[E];
-#pragma unroll
-      for (int e = 0; e < E; e++) {
-        const int k = lane + 64 * e;
-        bool neg;
-        int src;
-        if (r <= N) {
-          neg = k < r;
-          src = neg ? k - r + N : k - r;
-        } else {
-          const int rr = r - N;
-          neg = k >= rr;
-          src = neg ? k - rr : k - rr + N;
-        }
-        W v = acc[src];
-        // untraced: the reference's raw-word negation; traced: the NTT product
-        // ACC * X^r (binary.rs:158-181) is exact, i.e. negation of the canonical value
-        if (neg) v = traced ? F::neg_raw(F::canon(v)) : F::neg_raw(v);
-        dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
-      }
-      // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
-      gadget_mac<F, E>(pa, pb, dst, A.br_len, A.br_bits,
-                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, twl, lane);
-    }
-    // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
-    // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
-    ntt_inv_wave<F, E>(pa, lds, itwl, lane);
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      const int k = lane + 64 * e;
-      outA[k] = F::add(accA[k], pa[e]);
-    }
-    ntt_inv_wave<F, E>(pb, lds, itwl, lane);
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-      const int k = lane + 64 * e;
-      outB[k] = F::add(accB[k], pb[e]);
-    }
-    gate_sync();
+    W v = acc[src];
+    if (neg) v = traced ? F::neg_raw(F::canon(v)) : F::neg_raw(v);
+    dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
   }
+}
 
-  // 4. acc.b[0] += Q/8 (evaluate.rs:52) and key switching.
+// Step 4 by one wave: acc.b[0] += Q/8 (evaluate.rs:52), key switching and sample extraction.
+template <class F>
+__device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
+                                                  const typename DF<F>::W* accA,
+                                                  const typename DF<F>::W* accB,
+                                                  typename DF<F>::W* lds,
+                                                  const typename DF<F>::W* twl,
+                                                  const typename DF<F>::W* itwl, int lane) {
+  using W = typename DF<F>::W;
+  constexpr int E = 16;
+  constexpr int N = 64 * E;
+  const u32 n = A.n, n1 = A.n + 1;
+  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
+  const W* __restrict__ ksk = static_cast<const W*>(A.ksk);
+  const u32* wi = A.idx ? A.idx + 3 * (size_t)g : nullptr;
   const W b0 = F::add(accB[0], (W)A.q8);
   W src[E];
 #pragma unroll
@@ -358,6 +320,166 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   if (lane == 0) out[n] = (u64)bout;
 }
 
+// ----------------------------------------------------------------------------
+// Fused bootstrap, one wavefront per gate.
+// ----------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
+                                               unsigned short* ms,
+                                               const typename DF<F>::W* twl,
+                                               const typename DF<F>::W* itwl) {
+  using W = typename DF<F>::W;
+  constexpr int E = 16;
+  constexpr int N = 64 * E;
+  const int lane = threadIdx.x & 63;
+  const u32 n = A.n;
+  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
+  const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
+  // Untraced: one accumulator per gate, updated in place. Traced (the vfhe path): an
+  // (n+1)-slot history per gate; round i reads slot i and writes slot i+1, so the
+  // history is the core trace (the acc entering every round, then the final acc).
+  const bool traced = A.trace != nullptr;
+  W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N
+                   : static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  W* accB = accA + N;
+
+  gate_modswitch<F>(A, g, lane, 64, ms);
+  wave_sync();
+  gate_lut_init<F>(A, lane, 64, ms, accA, accB);
+  gate_sync();
+
+  // 3. blind rotation: for a_i != 0, ACC += ExtProd((X^a_i - 1) ACC, RGSW(s_i)).
+  const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
+  const size_t step = traced ? 2 * N : 0;
+  for (u32 i = 0; i < n; i++, accA += step, accB += step) {
+    const int r = ms[i];
+    W* outA = accA + step;
+    W* outB = accB + step;
+    if (r == 0) {  // binary.rs:70 skips the round; the vfhe path adds a zero product
+      if (traced) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int k = lane + 64 * e;
+          outA[k] = accA[k];
+          outB[k] = accB[k];
+        }
+        gate_sync();
+      }
+      continue;
+    }
+    const W* key = bsk + (size_t)i * rgsw_words;
+    W pa[E], pb[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+#pragma unroll 1
+    for (int comp = 0; comp < 2; comp++) {
+      W dst[E];
+      rotate_sub<F, E>(comp ? accB : accA, r, traced, lane, dst);
+      // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
+      gadget_mac<F, E>(pa, pb, dst, A.br_len, A.br_bits,
+                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, twl, lane);
+    }
+    // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
+    // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
+    ntt_inv_wave<F, E>(pa, lds, itwl, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = lane + 64 * e;
+      outA[k] = F::add(accA[k], pa[e]);
+    }
+    ntt_inv_wave<F, E>(pb, lds, itwl, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      const int k = lane + 64 * e;
+      outB[k] = F::add(accB[k], pb[e]);
+    }
+    gate_sync();
+  }
+  keyswitch_extract<F>(A, g, accA, accB, lds, twl, itwl, lane);
+}
+
+// ----------------------------------------------------------------------------
+// Latency mode: one gate per workgroup of 4 wavefronts. Wave w = 2c + h takes component c
+// of the rotated accumulator and half h of its gadget digits, accumulating a partial
+// NTT-domain product; the four partials meet in LDS, wave 0 finishes the a half (sum,
+// inverse NTT, accumulator add) and wave 1 the b half. Wave 0 then key-switches.
+// Untraced semantics only.
+// ----------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
+                                                     typename DF<F>::W* lds,
+                                                     unsigned short* ms,
+                                                     typename DF<F>::W* part,
+                                                     const typename DF<F>::W* twl,
+                                                     const typename DF<F>::W* itwl) {
+  using W = typename DF<F>::W;
+  constexpr int E = 16;
+  constexpr int N = 64 * E;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int comp = wave >> 1, half = wave & 1;
+  const u32 n = A.n;
+  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
+  const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
+  W* accA = static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  W* accB = accA + N;
+
+  gate_modswitch<F>(A, g, tid, 256, ms);
+  __syncthreads();
+  gate_lut_init<F>(A, tid, 256, ms, accA, accB);
+  __syncthreads();
+
+  const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
+  // this wave's digits: [j0, j1) of its component's gadget
+  const u32 j0 = half ? (A.br_len + 1) / 2 : 0, j1 = half ? A.br_len : (A.br_len + 1) / 2;
+  W* mine = part + (size_t)wave * 2 * N;
+  for (u32 i = 0; i < n; i++) {
+    const int r = ms[i];
+    if (r == 0) continue;  // uniform across the workgroup
+    const W* key = bsk + (size_t)i * rgsw_words + (size_t)comp * A.br_len * 2 * N;
+    W pa[E], pb[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+    {
+      W dst[E];
+      rotate_sub<F, E>(comp ? accB : accA, r, false, lane, dst);
+      // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
+      if (j0 > 0) {
+#pragma unroll
+        for (int e = 0; e < E; e++) dst[e] >>= j0 * A.br_bits;
+      }
+      gadget_mac<F, E>(pa, pb, dst, j1 - j0, A.br_bits, key + (size_t)j0 * 2 * N, lds, tw, twl,
+                       lane);
+    }
+    // partial products to LDS, [poly][e][lane]
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      mine[e * 64 + lane] = pa[e];
+      mine[N + e * 64 + lane] = pb[e];
+    }
+    __syncthreads();
+    if (wave < 2) {
+      // wave 0 sums the a partials, wave 1 the b partials
+      W* acc = wave ? accB : accA;
+      W s[E];
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        W v = part[(size_t)0 * 2 * N + wave * N + e * 64 + lane];
+#pragma unroll
+        for (int w = 1; w < 4; w++) v = DF<F>::add(v, part[(size_t)w * 2 * N + wave * N + e * 64 + lane]);
+        s[e] = v;
+      }
+      ntt_inv_wave<F, E>(s, lds, itwl, lane);
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        const int k = lane + 64 * e;
+        acc[k] = F::add(acc[k], s[e]);
+      }
+    }
+    __syncthreads();
+  }
+  if (wave == 0) keyswitch_extract<F>(A, g, accA, accB, lds, twl, itwl, lane);
+}
+
 // Copies the forward and inverse twiddle tables (N words each) into LDS for the
 // per-lane reads of NTT passes 2 and 3.
 template <class W>
@@ -387,6 +509,18 @@ void bootstrap_kernel(BootArgs A) {
   const u32 g = blockIdx.x * GPW + w;
   if (g >= A.count) return;  // after the only workgroup barrier
   bootstrap_gate<F>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+}
+
+// Latency-mode launch: one gate per workgroup of 4 wavefronts (bootstrap_gate_split).
+template <class F>
+__global__ __launch_bounds__(256) void bootstrap_split_kernel(BootArgs A) {
+  using W = typename DF<F>::W;
+  __shared__ W lds[4][1024 + 64];
+  __shared__ unsigned short ms[1024];
+  __shared__ W twl[2][1024];
+  __shared__ W part[4 * 2 * 1024];
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 256);
+  bootstrap_gate_split<F>(A, blockIdx.x, lds[threadIdx.x >> 6], ms, part, twl[0], twl[1]);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -544,6 +678,14 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
     hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2>), g4, dim3(256), 0, s, a);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStream_t s) {
+  if (is_goldilocks)
+    hipLaunchKernelGGL(bootstrap_split_kernel<Gl>, dim3(a.count), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL(bootstrap_split_kernel<Bb>, dim3(a.count), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -46,6 +46,8 @@ struct BootArgs {
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
+// Latency mode: 4 wavefronts per gate (untraced gates only).
+hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);

@@ -407,6 +407,17 @@ def set_tuned_config(set_id: int, gates_per_workgroup: int, waves_per_simd: int)
     check(lib().hb_set_tuned_config(set_id, gates_per_workgroup, waves_per_simd))
 
 
+def latency_mode() -> int:
+    """Largest batch that runs 4 wavefronts per gate (0: never)."""
+    v = C.c_uint32()
+    check(lib().hb_latency_mode(C.byref(v)))
+    return v.value
+
+
+def set_latency_mode(max_gates: int) -> None:
+    check(lib().hb_set_latency_mode(max_gates))
+
+
 def mixed_batch(parts) -> list[np.ndarray]:
     """Bootstrap gates of several evaluation keys (any parameter sets) in ONE launch.
 

@@ -110,3 +110,21 @@ def test_autotune_picks_a_variant_and_caches(hb, gold, tmp_path, monkeypatch):
     assert min(r["ms"], key=r["ms"].get) == best
     assert hb.tuned_config(hb.SET_GOLDILOCKS) == (*best, True)
     assert f"1 {best[0]} {best[1]}" in cache.read_text().splitlines()
+
+
+def test_latency_mode_matches_wave_per_gate(hb, orc, gold):
+    """The 4-waves-per-gate kernel and the 1-wave kernel give the same words."""
+    P, sk, bsk, ksk, ev = gold
+    count = 70
+    b0, b1, c0, c1 = _inputs(hb, sk, 77, count)
+    old = hb.latency_mode()
+    try:
+        hb.set_latency_mode(1 << 20)
+        split = ev.gate_batch("nand", c0, c1)
+        hb.set_latency_mode(0)
+        single = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_latency_mode(old)
+    assert np.array_equal(split, single)
+    ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[:8], c1[:8], threads=8)
+    assert np.array_equal(split[:8], ref)
