@@ -541,7 +541,10 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
   HIP_TRY(hipEventRecord(E.ev0, s));
-  HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
+  if (first <= E.split_max)
+    HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first, s));
+  else
+    HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
   return HB_OK;
 }

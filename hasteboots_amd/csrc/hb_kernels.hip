@@ -545,6 +545,31 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
   }
 }
 
+// Latency-mode mixed launch: one gate per workgroup of 4 wavefronts, parts as in
+// mixed_kernel (untraced parts only).
+__global__ __launch_bounds__(256) void mixed_split_kernel(const BootArgs* __restrict__ parts,
+                                                          int nparts) {
+  __shared__ u64 lds[4][1024 + 64];
+  __shared__ unsigned short ms[1024];
+  __shared__ u64 twl[2][1024];
+  __shared__ u64 part[4 * 2 * 1024];
+  const u32 b = blockIdx.x;
+  int p = 0;
+  while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
+  const BootArgs A = parts[p];
+  const int g = (int)(b - A.first_block);
+  const int w = threadIdx.x >> 6;
+  if (A.is_goldilocks) {
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 256);
+    bootstrap_gate_split<Gl>(A, g, lds[w], ms, part, twl[0], twl[1]);
+  } else {
+    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 256);
+    bootstrap_gate_split<Bb>(A, g, reinterpret_cast<u32*>(lds[w]), ms,
+                             reinterpret_cast<u32*>(part), t, t + 1024);
+  }
+}
+
 // ----------------------------------------------------------------------------
 // Microbench: negacyclic NTT over `count` polynomials, one wave per polynomial,
 // 4 waves per workgroup. Forward: natural order in, slot order out. Inverse: slot
@@ -693,6 +718,12 @@ hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int w
                             hipStream_t s) {
   return is_goldilocks ? launch_bootstrap_f<Gl>(a, gpw, wps, s)
                        : launch_bootstrap_f<Bb>(a, gpw, wps, s);
+}
+
+hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(mixed_split_kernel, dim3(total_gates), dim3(256), 0, s, parts_dev, nparts);
+  return hipGetLastError();
 }
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {

@@ -49,6 +49,9 @@ hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int w
 // Latency mode: 4 wavefronts per gate (untraced gates only).
 hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
+// Latency-mode mixed launch (4 wavefronts per gate; untraced parts only).
+hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
+                              hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
