@@ -124,6 +124,37 @@ def profiled_valu(set_name: str, gates: int):
     return (int(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
+def mixed_leg(hb, seed: int = 41) -> dict:
+    """Config 4: one launch bootstrapping NAND gates of BabyBear, Goldilocks and Zama keys
+    together; per-set counts ~ Geometric(mean 300) truncated at 2048, uniform bits.
+    Host buffers in and out (hb_mixed_batch); the kernel time is the launch's HIP events."""
+    rng = np.random.default_rng([seed, 4])
+    counts = [int(min(2048, rng.geometric(1 / 300))) for _ in range(3)]
+    parts, checks = [], []
+    for set_id, count in zip((0, 1, 2), counts):
+        P = hb.Parameters.of(set_id)
+        sk = hb.KeyGen.generate_secret_key(P, seed=100 + set_id)
+        ev = hb.Evaluator(sk)
+        enc = hb.Encryptor(sk)
+        b0, b1 = rng.integers(0, 2, size=(2, count))
+        parts.append((ev, "nand", enc.encrypt_many(b0), enc.encrypt_many(b1)))
+        checks.append((hb.Decryptor(sk), b0, b1))
+    hb.mixed_batch(parts)  # warm-up (kernel load, scratch)
+    t = time.perf_counter()
+    outs = hb.mixed_batch(parts)
+    wall = time.perf_counter() - t
+    kms = hb.last_kernel_ms()
+    wrong = 0
+    for (dec, b0, b1), out in zip(checks, outs):
+        wrong += sum(dec.decrypt(hb.LWECiphertext(out[i])) != 1 - (int(b0[i]) & int(b1[i]))
+                     for i in range(out.shape[0]))
+    total = sum(counts)
+    return {"counts": {"babybear": counts[0], "goldilocks": counts[1], "zama": counts[2]},
+            "gates": total, "kernel_ms": round(kms, 3),
+            "gates_per_s_kernel": round(total / (kms * 1e-3), 1),
+            "gates_per_s_end_to_end": round(total / wall, 1), "decrypt_mismatches": int(wrong)}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -154,6 +185,7 @@ def main() -> int:
                     help="skip the gates-per-workgroup autotuner before the batched leg")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of replays of the captured hipGraph")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter leg")
     ap.add_argument("--no-micro", action="store_true",
                     help="skip the NTT / external-product / MAC-peak microbenchmarks")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -265,6 +297,21 @@ def main() -> int:
         oracle_ok = bool(np.array_equal(ref, out[:k]))
     log(f"decrypt mismatches {wrong}/{G}; oracle bit-exact on {args.check}: {oracle_ok}")
 
+    # end to end: host ciphertexts in, host results out (PCIe both ways), one call
+    e2e = None
+    if args.steps > 0:
+        out_h = np.zeros_like(c0)
+        ev.gate_batch("nand", c0, c1)  # scratch sized
+        s = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            out_h = ev.gate_batch("nand", c0, c1)
+        el = time.perf_counter() - s
+        e2e = {"value": round(G * reps / el, 2), "unit": "bootstraps/s",
+               "includes": "H2D of both inputs, kernel, D2H of outputs (hb_gate_batch)",
+               "identical_to_device_path": bool(np.array_equal(out_h, out))}
+        log(f"end to end (host buffers): {e2e['value']} bootstraps/s")
+
     # plain path: one gate per launch
     lat = []
     for i in range(args.plain):
@@ -311,6 +358,11 @@ def main() -> int:
                    "latency": "circuit = submit to outputs on host; gate = inputs ready to "
                               "launch done", "sweep": sweep}
         del clients
+
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_leg(hb)
+        log(f"mixed batch: {mixed}")
 
     micro = None
     if not args.no_micro:
@@ -395,6 +447,8 @@ def main() -> int:
                  else None,
                  "mac_peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
                                     "8192 wavefronts"},
+        "end_to_end": e2e,
+        "mixed_batch": mixed,
         "microbench": micro,
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
                     "oracle_gates": args.check},
