@@ -36,6 +36,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "hb_gl_asm.inc")
 
 TEMP_BASE = 232  # v232..v255: 12 per chain
+# One zero-extension pair shared by both chains (one fewer mov per block, but the two
+# chains' products then run one after the other); measured slower on MI355X, so off.
+SHARED_Z = False
+# Preferred issue slots between a v_mad_u64_u32 and an instruction reading its result
+# (a soft scheduling constraint). On MI355X back-to-back dependent mads cost ~6.6 clocks
+# each and a gap of three instructions restores the full rate (4.2), but with two waves
+# per SIMD the other wave fills those stalls: a preference of 3 measured no faster than
+# none, so it is off.
+MAD_LATENCY = 0
 
 
 class Chain:
@@ -57,7 +66,9 @@ class Chain:
             return f"v{self.base + 10 + (0 if name == 'T1' else 1)}"
         m = re.fullmatch(r"([AUVWZ])(\.lo|\.hi)?", name)
         if m:
-            b = self.base + pairs[m.group(1)]
+            # Z (the zero-extension pair, high half kept 0) is shared by all chains
+            shared = SHARED_Z and m.group(1) == "Z"
+            b = (TEMP_BASE if shared else self.base) + pairs[m.group(1)]
             if m.group(2) == ".lo":
                 return f"v{b}"
             if m.group(2) == ".hi":
@@ -78,6 +89,20 @@ def halves(sym: str) -> list[str]:
     if re.fullmatch(r"[AUVWZ]", sym):
         return [sym + ".lo", sym + ".hi"]
     return [sym]
+
+
+def montmul_w(c: Chain, w0: str, w1: str, dst: str):
+    """dst <- montmul(y, w) with y in the fixed pair W (consumed): y0 * w0 and y0 * w1 are
+    formed first, y1 * w1 last, so W is rewritten only after its last read."""
+    c.add(f"v_mad_u64_u32 {{A}}, {{D}}, {{W.lo}}, {w0}, 0", ["A"], ["W.lo", w0])
+    c.add(f"v_mad_u64_u32 {{V}}, {{D}}, {{W.lo}}, {w1}, 0", ["V"], ["W.lo", w1])
+    c.add("v_mov_b32 {Z.lo}, {A.hi}", ["Z.lo"], ["A.hi"])
+    c.add(f"v_mad_u64_u32 {{U}}, {{D}}, {{W.hi}}, {w0}, {{Z}}", ["U"], ["W.hi", w0, "Z"])
+    c.add("v_mad_u64_u32 {V}, {D}, {U.lo}, 1, {V}", ["V"], ["U.lo", "V"])
+    c.add("v_mov_b32 {Z.lo}, {U.hi}", ["Z.lo"], ["U.hi"])
+    c.add(f"v_mad_u64_u32 {{W}}, {{D}}, {{W.hi}}, {w1}, {{Z}}", ["W"], ["W.hi", w1, "Z"])
+    c.add("v_mad_u64_u32 {W}, {D}, {V.hi}, 1, {W}", ["W"], ["V.hi", "W"])
+    monty(c, "A.lo", "V.lo", "W.lo", "W.hi", dst)
 
 
 def montmul(c: Chain, y0: str, y1: str, w0: str, w1: str, dst: str = "A"):
@@ -197,26 +222,20 @@ def gs_chain(tag: str, base: int, wkind: str) -> Chain:
     c = Chain(tag, base)
     x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
     w0, w1 = f"%{wkind}w{tag}0", f"%{wkind}w{tag}1"
-    # d = x - y mod P, canonical, into T1/T2 (via pair V)
-    gs_diff(c, x0, x1, y0, y1)
-    c.add("v_mov_b32 {T1}, {V.lo}", ["T1"], ["V.lo"])
-    c.add("v_mov_b32 {T2}, {V.hi}", ["T2"], ["V.hi"])
-    # y' = montmul(d, w) straight into the output pair; montmul rewrites T1/T2 only
-    # after its last multiply has read them
-    sub = Chain(tag, base)
-    montmul(sub, "{T1}", "{T2}", w0, w1, dst=f"%Y{tag}")
-    for text, defs, uses in sub.ins:
-        c.add(text, defs, ["T1" if u == "{T1}" else "T2" if u == "{T2}" else u for u in uses])
+    # d = x - y mod P, canonical, into W; y' = montmul(d, w) straight into the output
+    gs_diff(c, x0, x1, y0, y1, "W")
+    montmul_w(c, w0, w1, f"%Y{tag}")
     gs_sum(c, tag, x0, x1, y0, y1)
     return c
 
 
-def gs_diff(c: Chain, x0: str, x1: str, y0: str, y1: str):
-    """V <- x - y mod P, canonical for canonical x, y."""
-    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {y0}", ["V.lo", "S2"], [x0, y0])
-    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {y1}, {{S2}}", ["V.hi", "S2"], [x1, y1, "S2"])
+def gs_diff(c: Chain, x0: str, x1: str, y0: str, y1: str, r: str = "V"):
+    """Pair r <- x - y mod P, canonical for canonical x, y."""
+    c.add(f"v_sub_co_u32 {{{r}.lo}}, {{S2}}, {x0}, {y0}", [f"{r}.lo", "S2"], [x0, y0])
+    c.add(f"v_subb_co_u32 {{{r}.hi}}, {{S2}}, {x1}, {y1}, {{S2}}", [f"{r}.hi", "S2"],
+          [x1, y1, "S2"])
     c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
-    c.add("v_mad_i64_i32 {V}, {D}, {T2}, {%k55}, {V}", ["V"], ["T2", "%k55", "V"])
+    c.add(f"v_mad_i64_i32 {{{r}}}, {{D}}, {{T2}}, {{%k55}}, {{{r}}}", [r], ["T2", "%k55", r])
 
 
 def gs_sum(c: Chain, tag: str, x0: str, x1: str, y0: str, y1: str):
@@ -251,8 +270,10 @@ def schedule(chains: list[Chain]) -> list[str]:
     for ci, c in enumerate(chains):
         for ii, (text, defs, uses) in enumerate(c.ins):
             nodes.append((ci, ii))
-    # per-node dependence list: (pred node, min distance)
+    # per-node dependence list: (pred node, min distance); soft: preferred distances
+    # (result latency of v_mad_u64_u32, whose dependants stall when issued sooner)
     deps = {n: [] for n in nodes}
+    soft = {n: [] for n in nodes}
 
     def atoms(ci, syms):
         out = []
@@ -263,6 +284,11 @@ def schedule(chains: list[Chain]) -> list[str]:
             else:
                 out += [f"{ci}:{a}" for a in halves(s)]
         return out
+
+    # With SHARED_Z every Z access of chain k comes after every Z access of chain k-1
+    # (the chains' zero-extended products run one after the other).
+    def uses_z(ins):
+        return any(x.strip("{}").startswith("Z") for x in ins[1] + ins[2])
 
     for ci, c in enumerate(chains):
         for j, (tj, dj, uj) in enumerate(c.ins):
@@ -278,10 +304,17 @@ def schedule(chains: list[Chain]) -> list[str]:
                     valu_reader = not tj.startswith("s_")
                     dist = 3 if (sg and valu_writer and valu_reader) else 1
                     deps[(ci, j)].append(((ci, i), dist))
+                    if not sg and ti.startswith("v_mad_u64_u32"):
+                        soft[(ci, j)].append(((ci, i), MAD_LATENCY))
                 for r in (set(ai_use) & set(aj_def)) | (set(ai_def) & set(aj_def)):
                     if r == "D":
                         continue
                     deps[(ci, j)].append(((ci, i), 1))
+    for ci in range(1, len(chains) if SHARED_Z else 1):
+        prev = [(ci - 1, i) for i, ins in enumerate(chains[ci - 1].ins) if uses_z(ins)]
+        for j, ins in enumerate(chains[ci].ins):
+            if uses_z(ins):
+                deps[(ci, j)] += [(p, 1) for p in prev]
     # outputs (%X/%Y/%P defs) only after every read of any input operand
     in_readers = [n for n in nodes
                   if any(u.startswith("%") and u != "%k55"
@@ -292,10 +325,12 @@ def schedule(chains: list[Chain]) -> list[str]:
             for r in in_readers:
                 if r != n:
                     deps[n].append((r, 1))
-    # critical-path priority
+    # critical-path priority (with the soft latencies)
     succ = {n: [] for n in nodes}
     for n, ds in deps.items():
         for p, d in ds:
+            succ[p].append((n, d))
+        for p, d in soft[n]:
             succ[p].append((n, d))
     prio = {}
 
@@ -319,7 +354,10 @@ def schedule(chains: list[Chain]) -> list[str]:
             out.append(("s_nop 0", None))
             t += 1
             continue
-        n = max(ready, key=lambda n: (prio[n], -n[1], -n[0]))
+        def soft_ready(n):
+            return all(slot[p] + d <= t for p, d in soft[n])
+
+        n = max(ready, key=lambda n: (soft_ready(n), prio[n], -n[1], -n[0]))
         slot[n] = t
         remaining.discard(n)
         out.append((chains[n[0]].ins[n[1]][0], n[0]))
@@ -407,7 +445,8 @@ def verify(lines: list[str], n_out: int, vgpr_inputs: set[int]) -> None:
 def emit(name: str, chains: list[Chain], outputs: list[tuple[str, str, str]],
          inputs: list[tuple[str, str, str]], doc: str) -> str:
     """outputs/inputs: (symbol, constraint, C expression)."""
-    lines = [f"v_mov_b32 v{c.base + 9}, 0" for c in chains] + schedule(chains)
+    zeros = [TEMP_BASE + 9] if SHARED_Z else [c.base + 9 for c in chains]
+    lines = [f"v_mov_b32 v{z}, 0" for z in zeros] + schedule(chains)
     sg = []
     for c in chains:
         sg += [(f"s1{c.tag}", "=&s", f"s1{c.tag}"), (f"s2{c.tag}", "=&s", f"s2{c.tag}")]
