@@ -90,10 +90,10 @@ __device__ __forceinline__ void load4(const u32* __restrict__ p, u32 (&v)[4]) {
 // acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each. Goldilocks
 // accumulators stay in [0, 2^64) (the inverse transform canonicalises its input).
 // ntt_add_mul_assign_fast (algebra/src/polynomial/univariate/ntt_polynomial.rs:620-629).
-// Key words are software-pipelined one 4-slot group ahead: group 0 arrives in
-// (ka0, kb0), loaded by the caller before the forward transform, and group q+1 is in
-// flight while group q is multiplied. The scheduling barriers keep each load ahead of
-// the arithmetic it overlaps.
+// Key words are software-pipelined one 4-slot group ahead: group 0 arrives in (ka0, kb0),
+// loaded by the caller before the forward transform, and group q+1 is in flight while
+// group q is multiplied; the scheduling barriers keep each load ahead of the arithmetic it
+// overlaps.
 template <class F, int E>
 __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
                                         const typename DF<F>::W (&x)[E],
@@ -135,7 +135,7 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
 // Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
 // values, digit_j = (v >> j*bits) & mask. tw: forward twiddles in global memory (pass 1),
 // twv: the same table for per-lane reads (LDS copy).
-template <class F, int E>
+template <class F, int E, bool PF = true>
 __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
                                            const typename DF<F>::W (&src)[E], u32 len, u32 bits,
                                            const typename DF<F>::W* __restrict__ key,
@@ -155,7 +155,7 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
-    ntt_fwd_wave<F, E, true>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
+    ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
   }
 }
@@ -273,7 +273,7 @@ __device__ __forceinline__ void rotate_sub(const typename DF<F>::W* acc, int r, 
 }
 
 // Step 4 by one wave: acc.b[0] += Q/8 (evaluate.rs:52), key switching and sample extraction.
-template <class F>
+template <class F, bool PF = true>
 __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
                                                   const typename DF<F>::W* accA,
                                                   const typename DF<F>::W* accB,
@@ -297,14 +297,14 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
   W pa[E], pb[E];
 #pragma unroll
   for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
-  gadget_mac<F, E>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, twl, lane);
+  gadget_mac<F, E, PF>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, twl, lane);
   // b' = coefficient 0 of INTT(init.b + Σ) = b0 + Σ_slots Σ (1/N folded into the KSK).
   W s = 0;
 #pragma unroll
   for (int e = 0; e < E; e++) s = DF<F>::add(s, pb[e]);
   s = wave_sum<F>(s);
   const W bout = F::canon(DF<F>::add(F::canon(b0), s));
-  ntt_inv_wave<F, E>(pa, lds, itwl, lane);
+  ntt_inv_wave<F, E, PF>(pa, lds, itwl, lane);
   // extract_partial_lwe_locally (rlwe.rs:393-402): a' = [A0, -A_{N-1}, ..., -A_1][..n].
   u64* out = A.out + (size_t)(wi ? wi[2] : g) * n1;
 #pragma unroll
@@ -323,7 +323,9 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
 // ----------------------------------------------------------------------------
 // Fused bootstrap, one wavefront per gate.
 // ----------------------------------------------------------------------------
-template <class F>
+// PF: the transforms request their twiddles a stage ahead (one wave per SIMD); without it
+// they read them just before use, which leaves registers for two waves per SIMD.
+template <class F, bool PF = true>
 __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
                                                unsigned short* ms,
                                                const typename DF<F>::W* twl,
@@ -376,18 +378,18 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
       W dst[E];
       rotate_sub<F, E>(comp ? accB : accA, r, traced, lane, dst);
       // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
-      gadget_mac<F, E>(pa, pb, dst, A.br_len, A.br_bits,
+      gadget_mac<F, E, PF>(pa, pb, dst, A.br_len, A.br_bits,
                        key + (size_t)comp * A.br_len * 2 * N, lds, tw, twl, lane);
     }
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
     // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
-    ntt_inv_wave<F, E>(pa, lds, itwl, lane);
+    ntt_inv_wave<F, E, PF>(pa, lds, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
       outA[k] = F::add(accA[k], pa[e]);
     }
-    ntt_inv_wave<F, E>(pb, lds, itwl, lane);
+    ntt_inv_wave<F, E, PF>(pb, lds, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
@@ -395,7 +397,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
     }
     gate_sync();
   }
-  keyswitch_extract<F>(A, g, accA, accB, lds, twl, itwl, lane);
+  keyswitch_extract<F, PF>(A, g, accA, accB, lds, twl, itwl, lane);
 }
 
 // ----------------------------------------------------------------------------
@@ -508,7 +510,7 @@ void bootstrap_kernel(BootArgs A) {
   const int w = threadIdx.x >> 6;
   const u32 g = blockIdx.x * GPW + w;
   if (g >= A.count) return;  // after the only workgroup barrier
-  bootstrap_gate<F>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+  bootstrap_gate<F, WPS == 1>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
 }
 
 // Latency-mode launch: one gate per workgroup of 4 wavefronts (bootstrap_gate_split).

@@ -332,7 +332,10 @@ __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0
 // whose twiddle is psi^(N/2) = 2^48, is then exact without reduction:
 //   x + 2^48 y = (2^16 y) : x   and   x - 2^48 y + P = ~(2^16 y) : (x + 1)
 // as (high 32 bits : low 32 bits); both are valid words in [0, 2^64).
-template <class F, int E, bool SMALL = false>
+// PF: request per-lane twiddles a stage ahead (for one wave per SIMD, where nothing else
+// hides LDS latency); without it they are read just before use, which needs fewer
+// registers (for two waves per SIMD, which hide each other's latency).
+template <class F, int E, bool SMALL = false, bool PF = true>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
                                              const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
@@ -365,20 +368,23 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
   }
   // pass 2: block b = (lane + 64v) >> 2, stage s uses twv[(E << s) + (b << s) + j], j < 2^s
   W t2[V][4][8];
-  t2[0][0][0] = twv[E + (lane >> 2)];
+  if constexpr (PF) t2[0][0][0] = twv[E + (lane >> 2)];
   wave_sync();
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const int b = (lane + 64 * v) >> 2;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      if (s < 3) {
+      if constexpr (!PF) {
+#pragma unroll
+        for (int j = 0; j < (1 << s); j++) t2[v][s][j] = twv[(E << s) + (b << s) + j];
+      } else if (s < 3) {
 #pragma unroll
         for (int j = 0; j < (2 << s); j++) t2[v][s + 1][j] = twv[(E << (s + 1)) + (b << (s + 1)) + j];
       } else if (v + 1 < V) {
         t2[v + 1][0][0] = twv[E + ((lane + 64 * (v + 1)) >> 2)];
       }
-      sched_fence();
+      if constexpr (PF) sched_fence();
       const int half = 8 >> s;
 #pragma unroll
       for (int p = 0; p < 4; p++) {
@@ -386,7 +392,7 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
         Bfly<F>::ct2_v(x[16 * v + k0], x[16 * v + k0 + half], t2[v][s][k0 / (2 * half)],
                        x[16 * v + k1], x[16 * v + k1 + half], t2[v][s][k1 / (2 * half)]);
       }
-      sched_fence();
+      if constexpr (PF) sched_fence();
     }
   }
 #pragma unroll
@@ -400,19 +406,23 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
     for (int r = 0; r < 4; r++) x[4 * q + r] = lds[b3 + 272 * q + r];
   }
-  // pass 3 twiddles, requested behind the exchange reads
+  // pass 3 twiddles, requested behind the exchange reads (PF) or per group
   W t3[E / 4][3];
-#pragma unroll
-  for (int q = 0; q < E / 4; q++) {
+  auto load3 = [&](int q) {
     const int c = lane + 64 * q;
     t3[q][0] = twv[16 * E + c];
     t3[q][1] = twv[32 * E + 2 * c];
     t3[q][2] = twv[32 * E + 2 * c + 1];
+  };
+  if constexpr (PF) {
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) load3(q);
   }
   wave_sync();
   // pass 3
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
+    if constexpr (!PF) load3(q);
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
   }
@@ -421,7 +431,7 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 // Inverse transform without the 1/N factor. In: slot layout of ntt_fwd_wave.
 // Out: x[e] = position lane + 64e, canonical. itwv = psi^-bitrev(k) for passes 3 and 2
 // (normally an LDS copy); pass 1 uses the compile-time twiddles.
-template <class F, int E>
+template <class F, int E, bool PF = true>
 __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
                                              const typename DF<F>::W* itwv, int lane) {
   using W = typename DF<F>::W;
@@ -429,19 +439,23 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
   constexpr int V = E / 16;
   const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
   W t3[E / 4][3];
-#pragma unroll
-  for (int q = 0; q < E / 4; q++) {
+  auto load3 = [&](int q) {
     const int c = lane + 64 * q;
     t3[q][0] = itwv[16 * E + c];
     t3[q][1] = itwv[32 * E + 2 * c];
     t3[q][2] = itwv[32 * E + 2 * c + 1];
+  };
+  if constexpr (PF) {
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) load3(q);
+    sched_fence();
   }
-  sched_fence();
 #pragma unroll
   for (int e = 0; e < E; e++) x[e] = Bfly<F>::inv_in(x[e]);
   // pass 3
 #pragma unroll
   for (int q = 0; q < E / 4; q++) {
+    if constexpr (!PF) load3(q);
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
   }
@@ -458,15 +472,20 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
   }
   // pass 2, stages 3..0; stage s uses itwv[(E << s) + (b << s) + j], j < 2^s
   W t2[V][4][8];
+  if constexpr (PF) {
 #pragma unroll
-  for (int j = 0; j < 8; j++) t2[0][3][j] = itwv[(E << 3) + ((lane >> 2) << 3) + j];
+    for (int j = 0; j < 8; j++) t2[0][3][j] = itwv[(E << 3) + ((lane >> 2) << 3) + j];
+  }
   wave_sync();
 #pragma unroll
   for (int v = 0; v < V; v++) {
     const int b = (lane + 64 * v) >> 2;
 #pragma unroll
     for (int s = 3; s >= 0; s--) {
-      if (s > 0) {
+      if constexpr (!PF) {
+#pragma unroll
+        for (int j = 0; j < (1 << s); j++) t2[v][s][j] = itwv[(E << s) + (b << s) + j];
+      } else if (s > 0) {
 #pragma unroll
         for (int j = 0; j < (1 << (s - 1)); j++)
           t2[v][s - 1][j] = itwv[(E << (s - 1)) + (b << (s - 1)) + j];
@@ -475,7 +494,7 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
 #pragma unroll
         for (int j = 0; j < 8; j++) t2[v + 1][3][j] = itwv[(E << 3) + (bn << 3) + j];
       }
-      sched_fence();
+      if constexpr (PF) sched_fence();
       const int half = 8 >> s;
 #pragma unroll
       for (int p = 0; p < 4; p++) {
@@ -483,7 +502,7 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
         Bfly<F>::gs2_v(x[16 * v + k0], x[16 * v + k0 + half], t2[v][s][k0 / (2 * half)],
                        x[16 * v + k1], x[16 * v + k1 + half], t2[v][s][k1 / (2 * half)]);
       }
-      sched_fence();
+      if constexpr (PF) sched_fence();
     }
   }
 #pragma unroll
