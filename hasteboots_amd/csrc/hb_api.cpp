@@ -209,7 +209,7 @@ int ensure_acc(hb_ek* ek, size_t count) {
   ek->acc_dev = nullptr;
   ek->acc_cap = 0;
   const size_t wb = ek->p.is_goldilocks ? 8 : 4;
-  HIP_TRY(hipMalloc(&ek->acc_dev, count * 2 * ek->p.N * wb));
+  HIP_TRY(hipMalloc(&ek->acc_dev, count * kScratchWordsPerGate * wb));
   ek->acc_cap = count;
   return HB_OK;
 }
@@ -510,7 +510,7 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
   };
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
   std::lock_guard<std::mutex> g(E.mu);
-  const size_t per_gate = (size_t)2 * 1024 * 8;
+  const size_t per_gate = kScratchWordsPerGate * 8;
   if (E.mixed_acc_bytes < total * per_gate) {
     if (E.mixed_acc) HIP_TRY(hipFree(E.mixed_acc));
     E.mixed_acc = nullptr;
@@ -605,6 +605,8 @@ int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const
   Engine& E = engine();
   std::lock_guard<std::mutex> g(ek->mu);
   int rc = ensure_io(ek, count);
+  if (rc) return rc;
+  rc = ensure_acc(ek, count);  // digit scratch (the accumulators live in the trace)
   if (rc) return rc;
   const ParamSet& p = ek->p;
   if (lut) {
@@ -817,7 +819,7 @@ const char* hb_last_error(void) { return g_err.c_str(); }
 // ----------------------------------------------------------------------------
 struct hb_batch_graph {
   hb_ek* ek = nullptr;
-  void* acc = nullptr;  // private accumulator scratch (count * 2N words)
+  void* acc = nullptr;  // private per-gate scratch (count * kScratchWordsPerGate words)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   size_t count = 0;
@@ -837,7 +839,7 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   g->ek = ek;
   g->count = count;
   const size_t wb = ek->p.is_goldilocks ? 8 : 4;
-  HIP_TRY(hipMalloc(&g->acc, count * 2 * ek->p.N * wb));
+  HIP_TRY(hipMalloc(&g->acc, count * kScratchWordsPerGate * wb));
   BootArgs a = make_args(ek, gate, count, c0s, c1s, c2s, nullptr, outs, g->acc);
   hipStream_t cs;
   HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));

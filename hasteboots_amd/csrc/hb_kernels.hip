@@ -8,8 +8,9 @@
 //        normalisation) and, for BabyBear, by R = 2^32 (Montgomery constant form).
 //   KSK  [j < ks_len][p][N slots], same scaling.
 //   Slot order is the forward-NTT output order (bit-reversed evaluation order).
-//   Per-gate scratch: the accumulator RLWE, 2N words, raw Goldilocks words
-//   exactly as the reference's add_reduce produces them.
+//   Per-gate scratch (kScratchWordsPerGate words): the accumulator RLWE, 2N words, raw
+//   Goldilocks words exactly as the reference's add_reduce produces them, then N words
+//   for the byte digits of the rotated accumulator (store_digits8).
 #include "hb_kernels.hpp"
 #include "hb_ntt.hpp"
 
@@ -160,6 +161,63 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
   }
 }
 
+// Byte digits of one rotated accumulator component (8-bit gadget digits of 64-bit words,
+// decompose_lsb_bits_inplace with bits = 8, native_polynomial.rs:340-348) in the layout the
+// digit loop reads: digit j of word k = lane + 64 (4m + b) is byte b of dword m of the 16
+// bytes at dig[64 j + lane]. Per group of four words and half (digits 0-3 from the low
+// dwords, 4-7 from the high), one byte permute per word pair interleaves two digits
+// (a0.bj, a1.bj, a0.bj+1, a1.bj+1) and two more gather each digit's four bytes.
+// v_perm_b32 byte selectors 0-3 pick bytes of its second source, 4-7 of its first.
+__device__ __forceinline__ void store_digits8(const u64 (&v)[16], uint4* dig, int lane) {
+  u32 d[8][4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const u32 a0 = (u32)(v[4 * m + 0] >> (32 * h)), a1 = (u32)(v[4 * m + 1] >> (32 * h));
+      const u32 a2 = (u32)(v[4 * m + 2] >> (32 * h)), a3 = (u32)(v[4 * m + 3] >> (32 * h));
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const u32 sel = q ? 0x07030602u : 0x05010400u;
+        const u32 t01 = __builtin_amdgcn_perm(a1, a0, sel);
+        const u32 t23 = __builtin_amdgcn_perm(a3, a2, sel);
+        d[4 * h + 2 * q][m] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+        d[4 * h + 2 * q + 1][m] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 8; j++) dig[64 * j + lane] = make_uint4(d[j][0], d[j][1], d[j][2], d[j][3]);
+}
+
+// gadget_mac for the Goldilocks blind-rotation gadget (8-bit digits), the digits read from
+// their byte-transposed copy (store_digits8): one 16-byte load per digit, issued a digit
+// ahead, in place of a 16-word source held in registers through the digit loop.
+template <int E, bool PF>
+__device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
+                                                const uint4* __restrict__ dig, u32 len,
+                                                const u64* __restrict__ key, u64* lds,
+                                                const u64* twv, int lane) {
+  static_assert(E == 16, "one 16-byte load holds a lane's 16 digits");
+  constexpr int N = 64 * E;
+  uint4 d = dig[lane];
+  for (u32 j = 0; j < len; j++) {
+    const u64* ka = key + (size_t)(2 * j) * N;
+    u64 ka0[4], kb0[4];
+    load4(ka + 4 * lane, ka0);
+    load4(ka + N + 4 * lane, kb0);
+    const uint4 dn = dig[64 * (j + 1 < len ? j + 1 : j) + lane];
+    __builtin_amdgcn_sched_barrier(0);
+    const u32 w[4] = {d.x, d.y, d.z, d.w};
+    u64 x[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) x[e] = (w[e >> 2] >> (8 * (e & 3))) & 0xFFu;
+    ntt_fwd_wave<Gl, E, true, PF>(x, lds, twv, lane);
+    mac_row<Gl, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
+    d = dn;
+  }
+}
+
 // Wave-wide sum mod P of one value per lane.
 template <class F>
 __device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
@@ -179,6 +237,11 @@ __device__ __forceinline__ void gate_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Goldilocks sets whose blind-rotation gadget is 8 digits of 8 bits (the byte-digit path).
+__host__ __device__ inline bool dig8_set(const BootArgs& A) {
+  return A.is_goldilocks && A.br_bits == 8 && A.br_len == 8;
 }
 
 // ----------------------------------------------------------------------------
@@ -325,7 +388,11 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
 // ----------------------------------------------------------------------------
 // PF: the transforms request their twiddles a stage ahead (one wave per SIMD); without it
 // they read them just before use, which leaves registers for two waves per SIMD.
-template <class F, bool PF = true>
+// DIG8 (Goldilocks with 8 blind-rotation digits of 8 bits, the launcher's choice): the
+// digit loop reads byte digits from the gate's scratch (gadget_mac_dig8). A separate
+// instantiation, so that the register-held digit source of the other sets does not
+// weigh on its register allocation.
+template <class F, bool PF = true, bool DIG8 = false>
 __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
                                                unsigned short* ms,
                                                const typename DF<F>::W* twl,
@@ -341,9 +408,11 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   // (n+1)-slot history per gate; round i reads slot i and writes slot i+1, so the
   // history is the core trace (the acc entering every round, then the final acc).
   const bool traced = A.trace != nullptr;
-  W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N
-                   : static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  W* const scratch = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
+  W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N : scratch;
   W* accB = accA + N;
+  uint4* const dig = reinterpret_cast<uint4*>(scratch + 2 * N);
+  static_assert(!DIG8 || F::IS_GOLDILOCKS, "byte digits of 64-bit words");
 
   gate_modswitch<F>(A, g, lane, 64, ms);
   wave_sync();
@@ -378,8 +447,14 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
       W dst[E];
       rotate_sub<F, E>(comp ? accB : accA, r, traced, lane, dst);
       // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
-      gadget_mac<F, E, PF>(pa, pb, dst, A.br_len, A.br_bits,
-                       key + (size_t)comp * A.br_len * 2 * N, lds, tw, twl, lane);
+      const W* ckey = key + (size_t)comp * A.br_len * 2 * N;
+      if constexpr (DIG8) {
+        store_digits8(dst, dig, lane);
+        gate_sync();
+        gadget_mac_dig8<E, PF>(pa, pb, dig, A.br_len, ckey, lds, twl, lane);
+      } else {
+        gadget_mac<F, E, PF>(pa, pb, dst, A.br_len, A.br_bits, ckey, lds, tw, twl, lane);
+      }
     }
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
     // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
@@ -422,7 +497,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   const u32 n = A.n;
   const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
-  W* accA = static_cast<W*>(A.acc) + (size_t)g * 2 * N;
+  W* accA = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
   W* accB = accA + N;
 
   gate_modswitch<F>(A, g, tid, 256, ms);
@@ -499,7 +574,7 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
 // WPS: waves per SIMD the register allocation targets (1: up to 512 VGPR+AGPR per lane;
 // 2: at most 256, the excess spilled, so that two waves per SIMD hide each other's
 // LDS and memory latency).
-template <class F, int GPW, int WPS>
+template <class F, int GPW, int WPS, bool DIG8>
 __global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
 void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
@@ -510,7 +585,7 @@ void bootstrap_kernel(BootArgs A) {
   const int w = threadIdx.x >> 6;
   const u32 g = blockIdx.x * GPW + w;
   if (g >= A.count) return;  // after the only workgroup barrier
-  bootstrap_gate<F, WPS == 1>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+  bootstrap_gate<F, WPS == 1, DIG8>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
 }
 
 // Latency-mode launch: one gate per workgroup of 4 wavefronts (bootstrap_gate_split).
@@ -539,7 +614,10 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
   const int g = (int)(b - A.first_block);
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64);
-    bootstrap_gate<Gl>(A, g, lds, ms, twl[0], twl[1]);
+    if (dig8_set(A))
+      bootstrap_gate<Gl, true, true>(A, g, lds, ms, twl[0], twl[1]);
+    else
+      bootstrap_gate<Gl, true, false>(A, g, lds, ms, twl[0], twl[1]);
   } else {
     u32* t = reinterpret_cast<u32*>(&twl[0][0]);
     stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64);
@@ -690,19 +768,19 @@ __global__ __launch_bounds__(64) void modmul_peak_kernel(u64* __restrict__ out, 
 // ----------------------------------------------------------------------------
 // Launchers
 // ----------------------------------------------------------------------------
-template <class F>
+template <class F, bool D8>
 static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStream_t s) {
   const dim3 g1(a.count), g2((a.count + 1) / 2), g4((a.count + 3) / 4);
   if (gpw == 1 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 1, 1>), g1, dim3(64), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 1, 1, D8>), g1, dim3(64), 0, s, a);
   else if (gpw == 2 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 1>), g2, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 1, D8>), g2, dim3(128), 0, s, a);
   else if (gpw == 4 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 1>), g4, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 1, D8>), g4, dim3(256), 0, s, a);
   else if (gpw == 2 && wps == 2)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2>), g2, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2, D8>), g2, dim3(128), 0, s, a);
   else if (gpw == 4 && wps == 2)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2>), g4, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2, D8>), g4, dim3(256), 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -718,8 +796,9 @@ hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStrea
 
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s) {
-  return is_goldilocks ? launch_bootstrap_f<Gl>(a, gpw, wps, s)
-                       : launch_bootstrap_f<Bb>(a, gpw, wps, s);
+  if (!is_goldilocks) return launch_bootstrap_f<Bb, false>(a, gpw, wps, s);
+  return dig8_set(a) ? launch_bootstrap_f<Gl, true>(a, gpw, wps, s)
+                     : launch_bootstrap_f<Gl, false>(a, gpw, wps, s);
 }
 
 hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates,

@@ -28,7 +28,7 @@ struct BootArgs {
   const uint64_t* c2;  // majority only
   const uint64_t* lut; // null: built from (lut_mid, lut_lo, lut_hi)
   uint64_t* out;       // count*(n+1) canonical words
-  void* acc;           // count * 2N words of scratch
+  void* acc;           // count * kScratchWordsPerGate words of scratch
   double qf;           // (double)Q, as the reference's `as f64`
   uint64_t q8;         // Q >> 3 in the device word form
   uint64_t lut_lo, lut_hi;
@@ -41,6 +41,10 @@ struct BootArgs {
   const uint32_t* idx;   // optional: per gate (in0, in1, out) wire rows of c0 / out
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
 };
+
+// Per-gate scratch in BootArgs::acc, in words of the set's size: the accumulator RLWE (2N
+// words) then the rotated accumulator's 8-bit gadget digits, byte-transposed (N words).
+constexpr size_t kScratchWordsPerGate = 3 * 1024;
 
 // Batched-kernel variants: gpw gates (wavefronts) per workgroup and wps waves per SIMD
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2)}.

@@ -141,7 +141,7 @@ struct hb_server {
 
 namespace {
 
-constexpr size_t kAccPerGate = (size_t)2 * 1024 * 8;
+constexpr size_t kAccPerGate = kScratchWordsPerGate * 8;
 constexpr uint32_t kMaxParts = 4096;
 
 int admit(hb_server* S) {
