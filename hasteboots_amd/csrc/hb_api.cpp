@@ -155,7 +155,7 @@ int hb::ensure_engine(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
-    E.split_max = E.simd_slots / 4;
+    E.split_max = 3 * (E.simd_slots / 4);
   }
   load_tune_cache(E);
   return HB_OK;

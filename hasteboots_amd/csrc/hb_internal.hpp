@@ -56,8 +56,11 @@ struct Engine {
   // doubling up waves on half of them.
   uint32_t simd_slots = 1024;
   // Batches of at most this many untraced gates run in latency mode (4 wavefronts per
-  // gate); by default a quarter of the SIMDs, so one such batch still fills the device.
-  uint32_t split_max = 256;
+  // gate, one workgroup per gate, one such workgroup resident per CU); by default three
+  // per CU. Measured on MI355X (256 CUs), Goldilocks: 256 gates 21 ms, 257-512 42 ms,
+  // 513-768 64 ms in latency mode against 67-71 ms with one wavefront per gate, which
+  // wins from 1024 gates (72 against 85 ms); the other parameter sets cross over alike.
+  uint32_t split_max = 768;
 };
 
 Engine& engine();
