@@ -64,7 +64,7 @@ const char* tune_cache_path() { return std::getenv("HB_AUTOTUNE_CACHE"); }
 const int kVariants[5][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 2}};
 
 // Variant used for a batch of `count` gates of parameter set `set`; gpw = 0 selects the
-// latency-mode kernel (4 wavefronts per gate).
+// latency-mode kernel (split_waves wavefronts per gate).
 void batch_variant(const Engine& E, int set, size_t count, int* gpw, int* wps) {
   if (count <= E.split_max) {
     *gpw = 0, *wps = 1;
@@ -291,7 +291,8 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   int gpw, wps;
   batch_variant(E, p.set, count, &gpw, &wps);
   if (gpw == 0)
-    HIP_TRY(launch_bootstrap_split(a, p.is_goldilocks, s));
+    HIP_TRY(launch_bootstrap_split(a, p.is_goldilocks, split_waves(E, p.is_goldilocks, count),
+                                   s));
   else
     HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
@@ -541,8 +542,11 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
   HIP_TRY(hipEventRecord(E.ev0, s));
+  bool all_gl = true;
+  for (const BootArgs& a : args) all_gl = all_gl && a.is_goldilocks;
   if (first <= E.split_max)
-    HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first, s));
+    HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first,
+                               split_waves(E, all_gl, first), s));
   else
     HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
@@ -847,7 +851,8 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   if (e == hipSuccess) {
     int gpw, wps;
     batch_variant(E, ek->p.set, count, &gpw, &wps);
-    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks, cs)
+    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks,
+                                                      split_waves(E, ek->p.is_goldilocks, count), cs)
                              : launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
     e = hipStreamEndCapture(cs, &g->graph);
     if (le != hipSuccess) e = le;

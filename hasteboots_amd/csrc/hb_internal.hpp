@@ -63,6 +63,15 @@ struct Engine {
   uint32_t split_max = 768;
 };
 
+// Wavefronts per gate in latency mode: 8 (two per SIMD, hiding each other's latency) for
+// the Goldilocks sets, and for any launch of at most one gate per CU; otherwise 4, for
+// BabyBear, whose 4-wave workgroups fit two to a CU. Measured on MI355X (kernel ms, 8 / 4
+// waves): Goldilocks 256 gates 19.6 / 21.3, 512 39.3 / 42.4; Zama 256 30.5 / 31.9;
+// BabyBear 256 8.6 / 9.9, 512 17.2 / 13.1.
+inline int split_waves(const Engine& E, bool all_goldilocks, size_t count) {
+  return all_goldilocks || count <= E.simd_slots / 4 ? 8 : 4;
+}
+
 Engine& engine();
 int ensure_engine(int device);
 

@@ -135,10 +135,11 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
 // Σ over one gadget of digit NTTs: for j < len, x = NTT(digit_j(src)), acc += x ⊙ key_j.
 // Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
 // values, digit_j = (v >> j*bits) & mask. tw: forward twiddles in global memory (pass 1),
-// twv: the same table for per-lane reads (LDS copy).
-template <class F, int E, bool PF = true>
+// twv: the same table for per-lane reads (LDS copy). S: the source word type (a field word,
+// or u32 for a window of at most 32 bits of digits).
+template <class F, int E, bool PF = true, class S = typename DF<F>::W>
 __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
-                                           const typename DF<F>::W (&src)[E], u32 len, u32 bits,
+                                           const S (&src)[E], u32 len, u32 bits,
                                            const typename DF<F>::W* __restrict__ key,
                                            typename DF<F>::W* lds,
                                            const typename DF<F>::W* __restrict__ tw,
@@ -155,7 +156,7 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
     W x[E];
     const u32 sh = j * bits;
 #pragma unroll
-    for (int e = 0; e < E; e++) x[e] = (src[e] >> sh) & mask;
+    for (int e = 0; e < E; e++) x[e] = (W)(src[e] >> sh) & mask;
     ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
   }
@@ -476,39 +477,47 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 }
 
 // ----------------------------------------------------------------------------
-// Latency mode: one gate per workgroup of 4 wavefronts. Wave w = 2c + h takes component c
-// of the rotated accumulator and half h of its gadget digits, accumulating a partial
-// NTT-domain product; the four partials meet in LDS, wave 0 finishes the a half (sum,
-// inverse NTT, accumulator add) and wave 1 the b half. Wave 0 then key-switches.
+// Latency mode: one gate per workgroup of WAVES = 4 or 8 wavefronts. Wave w takes component
+// c = w / Q of the rotated accumulator (Q = WAVES / 2 waves per component) and the q = w % Q
+// share of its gadget digits, [q l / Q, (q + 1) l / Q), accumulating a partial NTT-domain
+// product. The partials meet in LDS (with 8 waves, waves 4-7 first hand theirs to waves
+// 0-3); wave 0 finishes the a half (sum, inverse NTT, accumulator add) and wave 1 the b
+// half. Wave 0 then key-switches. With 8 waves each SIMD holds two waves of the gate, which
+// hide each other's latency, and a wave's digit window (at most 4 digits) fits 32 bits.
 // Untraced semantics only.
 // ----------------------------------------------------------------------------
-template <class F>
+template <class F, int WAVES>
 __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      typename DF<F>::W* lds,
                                                      unsigned short* ms,
                                                      typename DF<F>::W* part,
                                                      const typename DF<F>::W* twl,
                                                      const typename DF<F>::W* itwl) {
+  static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per gate");
   using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
+  constexpr int Q = WAVES / 2;
+  constexpr bool PF = WAVES == 4;  // two waves per SIMD: twiddles read just before use
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int comp = wave >> 1, half = wave & 1;
+  const int comp = wave / Q, q = wave % Q;
   const u32 n = A.n;
   const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
   W* accA = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
   W* accB = accA + N;
 
-  gate_modswitch<F>(A, g, tid, 256, ms);
+  gate_modswitch<F>(A, g, tid, 64 * WAVES, ms);
   __syncthreads();
-  gate_lut_init<F>(A, tid, 256, ms, accA, accB);
+  gate_lut_init<F>(A, tid, 64 * WAVES, ms, accA, accB);
   __syncthreads();
 
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
   // this wave's digits: [j0, j1) of its component's gadget
-  const u32 j0 = half ? (A.br_len + 1) / 2 : 0, j1 = half ? A.br_len : (A.br_len + 1) / 2;
-  W* mine = part + (size_t)wave * 2 * N;
+  const u32 j0 = (u32)q * A.br_len / Q, j1 = (u32)(q + 1) * A.br_len / Q;
+  // partial slots [4][a, b][e][lane]: wave w < 4 owns slot w (with 8 waves, wave w + 4
+  // hands its partial to wave w through the same slot first)
+  W* mine = part + (size_t)(wave & 3) * 2 * N;
   for (u32 i = 0; i < n; i++) {
     const int r = ms[i];
     if (r == 0) continue;  // uniform across the workgroup
@@ -520,18 +529,44 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       W dst[E];
       rotate_sub<F, E>(comp ? accB : accA, r, false, lane, dst);
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
-      if (j0 > 0) {
+      const W* kj = key + (size_t)j0 * 2 * N;
+      if constexpr (WAVES == 8) {
+        u32 win[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) dst[e] >>= j0 * A.br_bits;
+        for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
+        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, tw, twl, lane);
+      } else {
+        if (j0 > 0) {
+#pragma unroll
+          for (int e = 0; e < E; e++) dst[e] >>= j0 * A.br_bits;
+        }
+        gadget_mac<F, E, PF>(pa, pb, dst, j1 - j0, A.br_bits, kj, lds, tw, twl, lane);
       }
-      gadget_mac<F, E>(pa, pb, dst, j1 - j0, A.br_bits, key + (size_t)j0 * 2 * N, lds, tw, twl,
-                       lane);
     }
-    // partial products to LDS, [poly][e][lane]
+    if constexpr (WAVES == 8) {
+      if (wave >= 4) {
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-      mine[e * 64 + lane] = pa[e];
-      mine[N + e * 64 + lane] = pb[e];
+        for (int e = 0; e < E; e++) {
+          mine[e * 64 + lane] = pa[e];
+          mine[N + e * 64 + lane] = pb[e];
+        }
+      }
+      __syncthreads();
+      if (wave < 4) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          pa[e] = DF<F>::add(pa[e], mine[e * 64 + lane]);
+          pb[e] = DF<F>::add(pb[e], mine[N + e * 64 + lane]);
+        }
+      }
+    }
+    // partial products to LDS, [poly][e][lane] (each lane rewrites only what it read)
+    if (wave < 4) {
+#pragma unroll
+      for (int e = 0; e < E; e++) {
+        mine[e * 64 + lane] = pa[e];
+        mine[N + e * 64 + lane] = pb[e];
+      }
     }
     __syncthreads();
     if (wave < 2) {
@@ -545,7 +580,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
         for (int w = 1; w < 4; w++) v = DF<F>::add(v, part[(size_t)w * 2 * N + wave * N + e * 64 + lane]);
         s[e] = v;
       }
-      ntt_inv_wave<F, E>(s, lds, itwl, lane);
+      ntt_inv_wave<F, E, PF>(s, lds, itwl, lane);
 #pragma unroll
       for (int e = 0; e < E; e++) {
         const int k = lane + 64 * e;
@@ -554,7 +589,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
     }
     __syncthreads();
   }
-  if (wave == 0) keyswitch_extract<F>(A, g, accA, accB, lds, twl, itwl, lane);
+  if (wave == 0) keyswitch_extract<F, PF>(A, g, accA, accB, lds, twl, itwl, lane);
 }
 
 // Copies the forward and inverse twiddle tables (N words each) into LDS for the
@@ -588,16 +623,17 @@ void bootstrap_kernel(BootArgs A) {
   bootstrap_gate<F, WPS == 1, DIG8>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
 }
 
-// Latency-mode launch: one gate per workgroup of 4 wavefronts (bootstrap_gate_split).
-template <class F>
-__global__ __launch_bounds__(256) void bootstrap_split_kernel(BootArgs A) {
+// Latency-mode launch: one gate per workgroup of WAVES wavefronts (bootstrap_gate_split).
+template <class F, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A) {
   using W = typename DF<F>::W;
-  __shared__ W lds[4][1024 + 64];
+  __shared__ W lds[WAVES][1024 + 64];
   __shared__ unsigned short ms[1024];
   __shared__ W twl[2][1024];
   __shared__ W part[4 * 2 * 1024];
-  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 256);
-  bootstrap_gate_split<F>(A, blockIdx.x, lds[threadIdx.x >> 6], ms, part, twl[0], twl[1]);
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
+  bootstrap_gate_split<F, WAVES>(A, blockIdx.x, lds[threadIdx.x >> 6], ms, part, twl[0],
+                                 twl[1]);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -625,11 +661,12 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
   }
 }
 
-// Latency-mode mixed launch: one gate per workgroup of 4 wavefronts, parts as in
+// Latency-mode mixed launch: one gate per workgroup of WAVES wavefronts, parts as in
 // mixed_kernel (untraced parts only).
-__global__ __launch_bounds__(256) void mixed_split_kernel(const BootArgs* __restrict__ parts,
-                                                          int nparts) {
-  __shared__ u64 lds[4][1024 + 64];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
+    const BootArgs* __restrict__ parts, int nparts) {
+  __shared__ u64 lds[WAVES][1024 + 64];
   __shared__ unsigned short ms[1024];
   __shared__ u64 twl[2][1024];
   __shared__ u64 part[4 * 2 * 1024];
@@ -640,13 +677,13 @@ __global__ __launch_bounds__(256) void mixed_split_kernel(const BootArgs* __rest
   const int g = (int)(b - A.first_block);
   const int w = threadIdx.x >> 6;
   if (A.is_goldilocks) {
-    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 256);
-    bootstrap_gate_split<Gl>(A, g, lds[w], ms, part, twl[0], twl[1]);
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
+    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], ms, part, twl[0], twl[1]);
   } else {
     u32* t = reinterpret_cast<u32*>(&twl[0][0]);
-    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 256);
-    bootstrap_gate_split<Bb>(A, g, reinterpret_cast<u32*>(lds[w]), ms,
-                             reinterpret_cast<u32*>(part), t, t + 1024);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * WAVES);
+    bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), ms,
+                                    reinterpret_cast<u32*>(part), t, t + 1024);
   }
 }
 
@@ -786,11 +823,20 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStream_t s) {
-  if (is_goldilocks)
-    hipLaunchKernelGGL(bootstrap_split_kernel<Gl>, dim3(a.count), dim3(256), 0, s, a);
-  else
-    hipLaunchKernelGGL(bootstrap_split_kernel<Bb>, dim3(a.count), dim3(256), 0, s, a);
+hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
+                                  hipStream_t s) {
+  if (waves != 4 && waves != 8) return hipErrorInvalidValue;
+  if (waves == 4) {
+    if (is_goldilocks)
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4>), dim3(a.count), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4>), dim3(a.count), dim3(256), 0, s, a);
+  } else {
+    if (is_goldilocks)
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8>), dim3(a.count), dim3(512), 0, s, a);
+    else
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8>), dim3(a.count), dim3(512), 0, s, a);
+  }
   return hipGetLastError();
 }
 
@@ -801,9 +847,15 @@ hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int w
                      : launch_bootstrap_f<Gl, false>(a, gpw, wps, s);
 }
 
-hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates,
+hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates, int waves,
                               hipStream_t s) {
-  hipLaunchKernelGGL(mixed_split_kernel, dim3(total_gates), dim3(256), 0, s, parts_dev, nparts);
+  if (waves != 4 && waves != 8) return hipErrorInvalidValue;
+  if (waves == 4)
+    hipLaunchKernelGGL(mixed_split_kernel<4>, dim3(total_gates), dim3(256), 0, s, parts_dev,
+                       nparts);
+  else
+    hipLaunchKernelGGL(mixed_split_kernel<8>, dim3(total_gates), dim3(512), 0, s, parts_dev,
+                       nparts);
   return hipGetLastError();
 }
 

@@ -50,12 +50,13 @@ constexpr size_t kScratchWordsPerGate = 3 * 1024;
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
-// Latency mode: 4 wavefronts per gate (untraced gates only).
-hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, hipStream_t s);
+// Latency mode: `waves` (4 or 8) wavefronts per gate (untraced gates only).
+hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
+                                  hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
-// Latency-mode mixed launch (4 wavefronts per gate; untraced parts only).
+// Latency-mode mixed launch (`waves` = 4 or 8 wavefronts per gate; untraced parts only).
 hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
-                              hipStream_t s);
+                              int waves, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,

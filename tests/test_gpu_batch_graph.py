@@ -112,8 +112,49 @@ def test_autotune_picks_a_variant_and_caches(hb, gold, tmp_path, monkeypatch):
     assert f"1 {best[0]} {best[1]}" in cache.read_text().splitlines()
 
 
+_EVALUATORS = {}
+
+
+@pytest.mark.parametrize("set_id,count", [(0, 70), (0, 260), (2, 70), (1, 260)])
+def test_latency_mode_wave_counts(hb, orc, keyset, set_id, count):
+    """Latency mode runs 8 wavefronts per gate for the Goldilocks sets and for launches of
+    at most one gate per CU, 4 for larger BabyBear launches (260 gates is more than one per
+    CU on MI355X's 256). Both kernels give the words of the one-wave kernel and the oracle."""
+    P, sk, bsk, ksk = keyset(set_id)
+    say(f"latency mode, set {set_id}, {count} gates")
+    if set_id not in _EVALUATORS:
+        _EVALUATORS[set_id] = hb.Evaluator(keys=(bsk, ksk), params=P)
+    ev = _EVALUATORS[set_id]
+    b0, b1, c0, c1 = _inputs_for(hb, sk, set_id, 91, count)
+    old = hb.latency_mode()
+    try:
+        hb.set_latency_mode(1 << 20)
+        split = ev.gate_batch("nand", c0, c1)
+        hb.set_latency_mode(0)
+        single = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_latency_mode(old)
+    assert np.array_equal(split, single)
+    sample = np.r_[0:3, count - 3:count]
+    ref = orc.OracleKeys(set_id, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=6)
+    assert np.array_equal(split[sample], ref)
+    dec = hb.Decryptor(sk)
+    assert all(dec.decrypt(hb.LWECiphertext(split[i])) == 1 - (int(b0[i]) & int(b1[i]))
+               for i in range(count))
+
+
+def _inputs_for(hb, sk, set_id, seed, count):
+    key = (set_id, seed, count)
+    if key not in _INPUTS:
+        rng = stream(seed + 1000 * set_id, "latency-bits")
+        enc = hb.Encryptor(sk)
+        b0, b1 = rng.integers(0, 2, size=(2, count))
+        _INPUTS[key] = (b0, b1, enc.encrypt_many(b0), enc.encrypt_many(b1))
+    return _INPUTS[key]
+
+
 def test_latency_mode_matches_wave_per_gate(hb, orc, gold):
-    """The 4-waves-per-gate kernel and the 1-wave kernel give the same words."""
+    """The latency-mode kernel and the 1-wave kernel give the same words."""
     P, sk, bsk, ksk, ev = gold
     count = 70
     b0, b1, c0, c1 = _inputs(hb, sk, 77, count)
