@@ -291,8 +291,9 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   int gpw, wps;
   batch_variant(E, p.set, count, &gpw, &wps);
   if (gpw == 0)
-    HIP_TRY(launch_bootstrap_split(a, p.is_goldilocks, split_waves(E, p.is_goldilocks, count),
-                                   s));
+    HIP_TRY(launch_bootstrap_split(
+        a, p.is_goldilocks,
+        split_waves(E, p.is_goldilocks, count, split8_fits(p.br_len, p.br_bits)), s));
   else
     HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
@@ -542,11 +543,14 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
   HIP_TRY(hipEventRecord(E.ev0, s));
-  bool all_gl = true;
-  for (const BootArgs& a : args) all_gl = all_gl && a.is_goldilocks;
+  bool all_gl = true, narrow = true;
+  for (const BootArgs& a : args) {
+    all_gl = all_gl && a.is_goldilocks;
+    narrow = narrow && split8_fits(a.br_len, a.br_bits);
+  }
   if (first <= E.split_max)
     HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first,
-                               split_waves(E, all_gl, first), s));
+                               split_waves(E, all_gl, first, narrow), s));
   else
     HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
   HIP_TRY(hipEventRecord(E.ev1, s));
@@ -851,8 +855,9 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   if (e == hipSuccess) {
     int gpw, wps;
     batch_variant(E, ek->p.set, count, &gpw, &wps);
-    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks,
-                                                      split_waves(E, ek->p.is_goldilocks, count), cs)
+    const int waves = split_waves(E, ek->p.is_goldilocks, count,
+                                  split8_fits(ek->p.br_len, ek->p.br_bits));
+    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks, waves, cs)
                              : launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
     e = hipStreamEndCapture(cs, &g->graph);
     if (le != hipSuccess) e = le;

@@ -67,8 +67,14 @@ struct Engine {
 // the Goldilocks sets, and for any launch of at most one gate per CU; otherwise 4, for
 // BabyBear, whose 4-wave workgroups fit two to a CU. Measured on MI355X (kernel ms, 8 / 4
 // waves): Goldilocks 256 gates 19.6 / 21.3, 512 39.3 / 42.4; Zama 256 30.5 / 31.9;
-// BabyBear 256 8.6 / 9.9, 512 17.2 / 13.1.
-inline int split_waves(const Engine& E, bool all_goldilocks, size_t count) {
+// BabyBear 256 8.6 / 9.9, 512 17.2 / 13.1. With 8 waves a wave's share of a gadget (at
+// most ceil(len / 4) digits) is held in 32 bits, which every parameter set here meets
+// (`narrow` false falls back to 4 waves).
+inline bool split8_fits(uint32_t br_len, uint32_t br_bits) {
+  return (br_len + 3) / 4 * br_bits <= 32;
+}
+inline int split_waves(const Engine& E, bool all_goldilocks, size_t count, bool narrow = true) {
+  if (!narrow) return 4;
   return all_goldilocks || count <= E.simd_slots / 4 ? 8 : 4;
 }
 

@@ -28,6 +28,10 @@ def test_parameter_sets(hb, set_id, q, br_len, ks_len, br_bits):
     assert (P.br_len, P.ks_len, P.br_bits, P.ks_bits) == (br_len, ks_len, br_bits, 5)
     # Parameters::new validation: N a power of two, 2N | Q-1
     assert P.N & (P.N - 1) == 0 and (P.q - 1) % (2 * P.N) == 0
+    # kernel shape assumptions: a quarter of a gadget's digits fits 32 bits (8-wave latency
+    # mode), and Goldilocks with 8 digits of 8 bits takes the byte-digit path
+    assert -(-P.br_len // 4) * P.br_bits <= 32
+    assert (P.q > 2**32 and P.br_bits == 8 and P.br_len == 8) == (set_id == 1)
 
 
 def test_bad_arguments_return_errors(hb):
