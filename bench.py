@@ -197,7 +197,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine has no multi-GPU path "
-              "(gates shard across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.",
+              "(gates are independent and would shard across GPUs with no exchange). Run with "
+              "--gpus 1.",
               file=sys.stderr)
         return 2
 
