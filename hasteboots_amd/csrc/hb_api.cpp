@@ -850,8 +850,12 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   HIP_TRY(hipMalloc(&g->acc, count * kScratchWordsPerGate * wb));
   BootArgs a = make_args(ek, gate, count, c0s, c1s, c2s, nullptr, outs, g->acc);
   hipStream_t cs;
-  HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    (void)hipFree(g->acc);
+    return fail(HB_ERR_DEVICE, "capture stream: %s", hipGetErrorString(e));
+  }
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     int gpw, wps;
     batch_variant(E, ek->p.set, count, &gpw, &wps);

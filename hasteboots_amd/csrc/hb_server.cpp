@@ -377,6 +377,22 @@ void release_slot(Slot& sl) {
   sl = Slot{};
 }
 
+// Device resources of a new server; on failure the caller releases whatever was made.
+int server_alloc(hb_server* S, uint64_t hbm_budget, uint32_t max_gates) {
+  HIP_TRY(hipMalloc((void**)&S->arena_base, hbm_budget));
+  S->arena.init(hbm_budget);
+  for (Slot& sl : S->slots) {
+    HIP_TRY(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&sl.ev0));
+    HIP_TRY(hipEventCreate(&sl.ev1));
+    HIP_TRY(hipMalloc((void**)&sl.parts_dev, kMaxParts * sizeof(BootArgs)));
+    HIP_TRY(hipHostMalloc((void**)&sl.parts_host, kMaxParts * sizeof(BootArgs)));
+    HIP_TRY(hipMalloc(&sl.acc, (size_t)max_gates * kAccPerGate));
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
+  return HB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -397,17 +413,11 @@ int hb_server_create(int device, uint32_t max_gates, uint64_t hbm_budget, hb_ser
   if (hbm_budget + slot_bytes > fr)
     return fail(HB_ERR_CAPACITY, "HBM budget %llu exceeds free device memory %zu",
                 (unsigned long long)hbm_budget, fr);
-  HIP_TRY(hipMalloc((void**)&S->arena_base, hbm_budget));
-  S->arena.init(hbm_budget);
-  for (Slot& sl : S->slots) {
-    HIP_TRY(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&sl.ev0));
-    HIP_TRY(hipEventCreate(&sl.ev1));
-    HIP_TRY(hipMalloc((void**)&sl.parts_dev, kMaxParts * sizeof(BootArgs)));
-    HIP_TRY(hipHostMalloc((void**)&sl.parts_host, kMaxParts * sizeof(BootArgs)));
-    HIP_TRY(hipMalloc(&sl.acc, (size_t)max_gates * kAccPerGate));
+  rc = server_alloc(S.get(), hbm_budget, max_gates);
+  if (rc) {
+    (void)hb_server_destroy(S.release());  // frees what was allocated; keeps the error
+    return rc;
   }
-  HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
   S->st.budget = hbm_budget;
   hb_server* raw = S.release();
   raw->worker = std::thread(worker_loop, raw);
