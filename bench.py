@@ -178,7 +178,7 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="gates compared with the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-rates", default="1,2,4",
+    ap.add_argument("--serving-rates", default="1,2,4,8",
                     help="comma-separated circuit arrival rates (1/s) for the serving leg; "
                          "empty to skip")
     ap.add_argument("--serving-seconds", type=float, default=8.0)

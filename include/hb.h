@@ -241,8 +241,9 @@ typedef struct {
   double gate_p50_ms, gate_p99_ms;
 } hb_server_stats_t;
 
-/* max_gates_per_launch: gate slots per launch; hbm_budget_bytes: arena size (0 = half of
- * the free device memory, at most 32 GiB). */
+/* max_gates_per_launch: gate slots per launch (a launch takes at most one gate per compute
+ * unit, 256 on MI355X, whichever is smaller); hbm_budget_bytes: arena size (0 = half of the
+ * free device memory, at most 32 GiB). */
 int hb_server_create(int device, uint32_t max_gates_per_launch, uint64_t hbm_budget_bytes,
                      hb_server_t** out);
 int hb_server_destroy(hb_server_t* server);
