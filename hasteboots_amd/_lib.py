@@ -51,10 +51,10 @@ class HbServerStats(C.Structure):
 
 class HbTuneResult(C.Structure):
     _fields_ = [("gates_per_workgroup", C.c_int32), ("waves_per_simd", C.c_int32),
-                ("probe_gates", C.c_uint64), ("ms", C.c_double * 5)]
+                ("probe_gates", C.c_uint64), ("ms", C.c_double * 6)]
 
 
-TUNE_VARIANTS = [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2)]
+TUNE_VARIANTS = [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2), (4, 3)]
 
 
 class HbError(RuntimeError):

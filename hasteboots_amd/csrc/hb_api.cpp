@@ -61,7 +61,7 @@ namespace {
 const char* tune_cache_path() { return std::getenv("HB_AUTOTUNE_CACHE"); }
 
 // The batched-kernel variants, in the order of hb_tune_result_t::ms.
-const int kVariants[5][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 2}};
+const int kVariants[HB_TUNE_VARIANTS][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 2}, {4, 3}};
 
 // Variant used for a batch of `count` gates of parameter set `set`; gpw = 0 selects the
 // latency-mode kernel (split_waves wavefronts per gate).
@@ -930,7 +930,7 @@ int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out) {
   BootArgs a = make_args(ek, GATE_NAND, probe_gates, c0, c1, nullptr, nullptr, o, ek->acc_dev);
   double best = 1e300;
   int best_v = 0;
-  for (int v = 0; v < 5; v++) {
+  for (int v = 0; v < HB_TUNE_VARIANTS; v++) {
     const int gpw = kVariants[v][0], wps = kVariants[v][1];
     HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, E.stream));  // warm-up
     HIP_TRY(hipEventRecord(E.ev0, E.stream));

@@ -607,8 +607,8 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
 }
 
 // WPS: waves per SIMD the register allocation targets (1: up to 512 VGPR+AGPR per lane;
-// 2: at most 256, the excess spilled, so that two waves per SIMD hide each other's
-// LDS and memory latency).
+// 2: at most 256, 3: at most 168, the excess spilled, so that the waves of a SIMD hide each
+// other's LDS and memory latency; 3 suits BabyBear's 32-bit words).
 template <class F, int GPW, int WPS, bool DIG8>
 __global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
 void bootstrap_kernel(BootArgs A) {
@@ -818,6 +818,8 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
     hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2, D8>), g2, dim3(128), 0, s, a);
   else if (gpw == 4 && wps == 2)
     hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2, D8>), g4, dim3(256), 0, s, a);
+  else if (gpw == 4 && wps == 3)
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 3, D8>), g4, dim3(256), 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -47,7 +47,7 @@ struct BootArgs {
 constexpr size_t kScratchWordsPerGate = 3 * 1024;
 
 // Batched-kernel variants: gpw gates (wavefronts) per workgroup and wps waves per SIMD
-// targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2)}.
+// targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2), (4,3)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
 // Latency mode: `waves` (4 or 8) wavefronts per gate (untraced gates only).

@@ -140,17 +140,18 @@ int hb_batch_graph_destroy(hb_batch_graph_t* graph);
  * key's parameter set and keeps the fastest for that set (used by every later batch
  * launch of the set); when the environment variable HB_AUTOTUNE_CACHE names a file the
  * choice is stored there and a new process reads it back. Variants, in the order of
- * ms[]: (gates per workgroup, waves per SIMD) = (1,1), (2,1), (4,1), (2,2), (4,2).
- * Gates per workgroup share one LDS copy of the twiddle tables; 2 waves per SIMD caps
- * registers at 256 per lane so two waves hide each other's latency. The tuned variant
+ * ms[]: (gates per workgroup, waves per SIMD) = (1,1), (2,1), (4,1), (2,2), (4,2), (4,3).
+ * Gates per workgroup share one LDS copy of the twiddle tables; 2 (3) waves per SIMD cap
+ * registers at 256 (168) per lane so the waves hide each other's latency. The tuned variant
  * applies to batches larger than one wave per SIMD of the device (4 x compute units);
  * smaller batches always run (1,1). Probe with at least twice that many gates. No
  * reference counterpart. */
+#define HB_TUNE_VARIANTS 6
 typedef struct {
   int gates_per_workgroup;
   int waves_per_simd;
   uint64_t probe_gates;
-  double ms[5]; /* kernel time per variant */
+  double ms[HB_TUNE_VARIANTS]; /* kernel time per variant */
 } hb_tune_result_t;
 int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out);
 /* Current variant for a parameter set; *tuned = 1 if it came from hb_autotune, the cache

@@ -37,7 +37,7 @@ def _inputs(hb, sk, seed, count):
     return _INPUTS[(seed, count)]
 
 
-@pytest.mark.parametrize("variant", [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2)])
+@pytest.mark.parametrize("variant", [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2), (4, 3)])
 def test_variants_match_oracle(hb, orc, gold, variant):
     P, sk, bsk, ksk, ev = gold
     say(f"variant gates/workgroup, waves/SIMD = {variant}")
@@ -64,10 +64,10 @@ _VARIANT_OUT = {}
 
 
 def test_variants_agree():
-    """All five variants give the same words on the same batch (runs after them)."""
+    """All six variants give the same words on the same batch (runs after them)."""
     outs = list(_VARIANT_OUT.values())
-    if len(outs) < 5:
-        pytest.skip("needs the five variant runs of this module")
+    if len(outs) < 6:
+        pytest.skip("needs the six variant runs of this module")
     assert all(np.array_equal(o, outs[0]) for o in outs[1:])
 
 
@@ -151,6 +151,31 @@ def _inputs_for(hb, sk, set_id, seed, count):
         b0, b1 = rng.integers(0, 2, size=(2, count))
         _INPUTS[key] = (b0, b1, enc.encrypt_many(b0), enc.encrypt_many(b1))
     return _INPUTS[key]
+
+
+@pytest.mark.parametrize("variant", [(2, 2), (4, 3)])
+def test_babybear_variants_match_oracle(hb, orc, keyset, variant):
+    """BabyBear through the 2- and 3-waves-per-SIMD variants (more gates than SIMDs, the
+    last workgroup partly empty) against the oracle and decryption."""
+    P, sk, bsk, ksk = keyset(0)
+    say(f"BabyBear variant {variant}")
+    if 0 not in _EVALUATORS:
+        _EVALUATORS[0] = hb.Evaluator(keys=(bsk, ksk), params=P)
+    ev = _EVALUATORS[0]
+    count = 1024 + 9
+    b0, b1, c0, c1 = _inputs_for(hb, sk, 0, 93, count)
+    old = hb.tuned_config(hb.SET_BABYBEAR)
+    hb.set_tuned_config(hb.SET_BABYBEAR, *variant)
+    try:
+        out = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_tuned_config(hb.SET_BABYBEAR, old[0], old[1])
+    sample = np.r_[0:4, count - 4:count]
+    ref = orc.OracleKeys(0, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=8)
+    assert np.array_equal(out[sample], ref)
+    dec = hb.Decryptor(sk)
+    assert all(dec.decrypt(hb.LWECiphertext(out[i])) == 1 - (int(b0[i]) & int(b1[i]))
+               for i in range(count))
 
 
 def test_latency_mode_matches_wave_per_gate(hb, orc, gold):
