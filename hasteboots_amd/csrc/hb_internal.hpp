@@ -77,6 +77,9 @@ inline int split_waves(const Engine& E, bool all_goldilocks, size_t count, bool 
   if (!narrow) return 4;
   return all_goldilocks || count <= E.simd_slots / 4 ? 8 : 4;
 }
+// The mixed latency-mode kernel sizes its LDS for 64-bit words whatever the parts, so one
+// workgroup fits a CU with 4 waves or with 8: 8 whenever the digit windows allow.
+inline int mixed_split_waves(bool narrow) { return narrow ? 8 : 4; }
 
 Engine& engine();
 int ensure_engine(int device);

@@ -253,16 +253,15 @@ int launch(hb_server* S, Slot& sl) {
   HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
-  bool any_traced = false, all_gl = true, narrow = true;
+  bool any_traced = false, narrow = true;
   for (uint32_t k = 0; k < nparts; k++) {
     const BootArgs& a = sl.parts_host[k];
     any_traced |= a.trace != nullptr;
-    all_gl = all_gl && a.is_goldilocks;
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
   if (!any_traced && used <= engine().split_max)
-    HIP_TRY(launch_mixed_split(sl.parts_dev, (int)nparts, used,
-                               split_waves(engine(), all_gl, used, narrow), sl.s));
+    HIP_TRY(launch_mixed_split(sl.parts_dev, (int)nparts, used, mixed_split_waves(narrow),
+                               sl.s));
   else
     HIP_TRY(launch_mixed(sl.parts_dev, (int)nparts, used, sl.s));
   HIP_TRY(hipEventRecord(sl.ev1, sl.s));

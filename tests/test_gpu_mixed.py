@@ -56,6 +56,35 @@ def test_mixed_small_all_gates_bit_exact(hb, orc, evaluators):
             assert dec.decrypt(hb.LWECiphertext(out[i])) == expect
 
 
+def test_mixed_latency_mode_and_wave_per_gate_agree(hb, orc, evaluators):
+    """The same mixed launch through the latency-mode kernel (8 wavefronts per gate, the
+    default at this size) and the one-wavefront-per-gate kernel (latency mode off) gives the
+    same words, which the oracle pins on a sample."""
+    say("mixed: latency mode against one wave per gate")
+    rng = stream(17, "mixed-modes")
+    parts, meta = [], []
+    for s, gate, count in [(2, "nand", 40), (1, "or", 25), (0, "nand", 60)]:
+        P, sk, bsk, ksk, ev = evaluators[s]
+        enc = hb.Encryptor(sk)
+        b0, b1 = rng.integers(0, 2, size=(2, count))
+        c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+        parts.append((ev, gate, c0, c1, None))
+        meta.append((s, gate, c0, c1))
+    old = hb.latency_mode()
+    try:
+        hb.set_latency_mode(1 << 20)
+        split = hb.mixed_batch(parts)
+        hb.set_latency_mode(0)
+        single = hb.mixed_batch(parts)
+    finally:
+        hb.set_latency_mode(old)
+    for (s, gate, c0, c1), a, b in zip(meta, split, single):
+        assert np.array_equal(a, b), (s, gate)
+        P, sk, bsk, ksk, ev = evaluators[s]
+        ref = orc.OracleKeys(s, bsk, ksk).gate_batch(GATE_ID[gate], c0[:2], c1[:2], threads=2)
+        assert np.array_equal(a[:2], ref), (s, gate)
+
+
 def test_mixed_geometric_counts_config(hb, orc, evaluators):
     """Per-set counts ~ Geometric(mean 300) truncated at 2048, uniform bits, one launch.
     Every output decrypts to NAND; a seeded sample per set is compared word for word with
