@@ -126,23 +126,26 @@ struct Bb {
   static constexpr bool IS_GOLDILOCKS = false;
 
   HB_HD static u32 canon(u32 x) { return x; }
+  // Canonical inputs (< P < 2^31): the wrapped value of the wrong branch is >= 2^32 - P,
+  // above every canonical word, so an unsigned min picks the result (no compare and
+  // select, so no carry-flag dependence on the VALU).
+  HB_HD static u32 umin(u32 a, u32 b) { return a < b ? a : b; }
   HB_HD static u32 add(u32 a, u32 b) {
-    u32 s = a + b;
-    u32 t = s - P;
-    return s < P ? s : t;
+    const u32 s = a + b;
+    return umin(s, s - P);
   }
   HB_HD static u32 sub(u32 a, u32 b) {
-    u32 d = a - b;
-    return a < b ? d + P : d;
+    const u32 d = a - b;
+    return umin(d, d + P);
   }
   HB_HD static u32 neg_raw(u32 x) { return x == 0 ? 0 : P - x; }
   HB_HD static u32 sub_canon(u32 a, u32 b) { return sub(a, b); }
-  // x * y * R^-1 mod P for x*y < P * 2^32.
+  // x * y * R^-1 mod P for x*y < P * 2^32, canonical. t * P matches x in the low 32 bits,
+  // so (x - t P) / 2^32 = hi(x) - hi(t P) exactly, in (-P, P).
   HB_HD static u32 mont_reduce(u64 x) {
-    u32 t = (u32)x * MU;
-    u64 u = (u64)t * P;
-    u32 hi = (u32)((x - u) >> 32);
-    return x < u ? hi + P : hi;
+    const u32 t = (u32)x * MU;
+    const u32 r = (u32)(x >> 32) - (u32)(((u64)t * P) >> 32);
+    return umin(r, r + P);
   }
   HB_HD static u32 mont_mul(u32 a, u32 b) { return mont_reduce((u64)a * b); }
   // Host helpers (plain modular arithmetic).

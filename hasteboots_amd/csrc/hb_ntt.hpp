@@ -52,7 +52,18 @@ struct DF<Bb> {
   __device__ __forceinline__ static W add(W a, W b) { return Bb::add(a, b); }
   __device__ __forceinline__ static W sub(W a, W b) { return Bb::sub(a, b); }
   // Constants (twiddles, keys) are stored times R = 2^32.
-  __device__ __forceinline__ static W mul(W a, W c) { return Bb::mont_mul(a, c); }
+  // Bb::mont_mul with both products pinned to full 64 bits (the empty asm makes the whole
+  // product live), so they compile to v_mad_u64_u32 rather than v_mul_lo_u32 /
+  // v_mul_hi_u32, which issue slower on gfx950: 4096 BabyBear gates 74.9 -> 67.5 ms.
+  __device__ __forceinline__ static W mul(W a, W c) {
+    const u64 x = (u64)a * c;
+    u64 t = (u64)(u32)x * Bb::MU;
+    asm("" : "+v"(t));
+    u64 u = (u64)(u32)t * Bb::P;
+    asm("" : "+v"(u));
+    const u32 r = (u32)(x >> 32) - (u32)(u >> 32);  // low words cancel: x = u mod 2^32
+    return Bb::umin(r, r + Bb::P);
+  }
   __device__ __forceinline__ static W canon(W a) { return a; }
 };
 
