@@ -176,14 +176,25 @@ def test_bootstrap_custom_lut(hb, orc, keyset, set_id):
     assert np.array_equal(got.words, ok.bootstrap(c, lut))
 
 
-def test_goldilocks_raw_word_accumulator(hb, orc):
+@pytest.mark.parametrize("latency", [True, False], ids=["latency-mode", "wave-per-gate"])
+def test_goldilocks_raw_word_accumulator(hb, orc, latency):
     """Adversarial keys that drive the accumulator into raw words in [P, 2^64), so the
     reference's wrapping negation (ops.rs:69-71) of such words is exercised every round.
 
     Every RGSW is zero except c_m[0].b = Q-1 in all slots (the constant polynomial -1), so
     each round adds -digit_0((X^a - 1) ACC.b) to ACC.b; with a small-valued LUT the sums
-    land in [P, 2^64). GPU and oracle must agree word for word."""
-    say("raw-word accumulator")
+    land in [P, 2^64). GPU and oracle must agree word for word, both through the
+    latency-mode kernel and through the one-wavefront-per-gate kernel with byte digits."""
+    say(f"raw-word accumulator, latency mode {latency}")
+    old = hb.latency_mode()
+    hb.set_latency_mode(old if latency else 0)
+    try:
+        _raw_word_accumulator(hb, orc)
+    finally:
+        hb.set_latency_mode(old)
+
+
+def _raw_word_accumulator(hb, orc):
     set_id = hb.SET_GOLDILOCKS
     P = hb.Parameters.of(set_id)
     n, N, L = P.n, P.N, P.br_len
@@ -225,3 +236,28 @@ def test_modulus_switch_rounding_boundaries(hb, orc, keyset, set_id):
     got = ev.gate_batch("nand", base, zero)
     ref = ok.gate_batch(NAND, base, zero, threads=4)
     assert np.array_equal(got, ref)
+
+
+def test_zama_batch_through_tuned_variant(hb, orc, keyset):
+    """Zama-Goldilocks (13 five-bit digits, digits held in registers) through the (4, 2)
+    batched variant: more gates than SIMDs, the last workgroup partly empty."""
+    P, sk, bsk, ksk = keyset(2)
+    count = 1024 + 9
+    say(f"zama batch {count} through (4, 2)")
+    rng = stream(2, "zama-variant")
+    b0, b1 = rng.integers(0, 2, size=(2, count))
+    enc = hb.Encryptor(sk)
+    c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+    old = hb.tuned_config(hb.SET_ZAMA)
+    hb.set_tuned_config(hb.SET_ZAMA, 4, 2)
+    try:
+        out = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_tuned_config(hb.SET_ZAMA, old[0], old[1])
+    sample = np.r_[0:2, count - 2:count]
+    ref = orc.OracleKeys(2, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=4)
+    assert np.array_equal(out[sample], ref)
+    dec = hb.Decryptor(sk)
+    assert all(dec.decrypt(hb.LWECiphertext(out[i])) == 1 - (int(b0[i]) & int(b1[i]))
+               for i in range(count))
