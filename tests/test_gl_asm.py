@@ -1,7 +1,7 @@
 """CPU checks of the hand-scheduled Goldilocks sequences (hasteboots_amd/csrc/hb_gl_asm.inc).
 
 The generated file must match its generator, and every macro is run through a one-lane
-interpreter of the ten gfx950 instructions it uses, against Python big-integer
+interpreter of the eleven gfx950 instructions it uses, against Python big-integer
 arithmetic: CT and GS butterflies and the MAC with Montgomery-form constants
 (c stands for c * 2^-64 mod P), on random and edge words.
 """
@@ -119,6 +119,8 @@ class Lane:
                 self.wr32(a[0], self.rd32(a[2]) << (self.rd32(a[1]) & 31))
             elif mn == "v_lshrrev_b32":
                 self.wr32(a[0], self.rd32(a[2]) >> (self.rd32(a[1]) & 31))
+            elif mn == "v_lshrrev_b64":
+                self.wr64(a[0], self.rd64(a[2]) >> (self.rd32(a[1]) & 63))
             elif mn == "v_alignbit_b32":
                 v = (self.rd32(a[1]) << 32) | self.rd32(a[2])
                 self.wr32(a[0], v >> (self.rd32(a[3]) & 31))
