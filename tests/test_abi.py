@@ -43,6 +43,27 @@ def test_bad_arguments_return_errors(hb):
     assert l.hb_sk_create(9, 0, C.byref(h)) == -1
 
 
+def test_tuning_and_launch_arguments_are_checked_before_the_device(hb):
+    """Bad parameter sets, kernel variants and handles are refused with HB_ERR_ARG and a
+    message, without touching a GPU."""
+    from hasteboots_amd._lib import lib
+    l = lib()
+    assert l.hb_set_tuned_config(3, 4, 2) == -1
+    assert b"parameter set" in l.hb_last_error()
+    for gpw, wps in ((3, 1), (4, 4), (1, 2), (8, 2), (0, 1)):
+        assert l.hb_set_tuned_config(1, gpw, wps) == -1, (gpw, wps)
+        assert b"no kernel variant" in l.hb_last_error()
+    g, w = C.c_int(), C.c_int()
+    assert l.hb_tuned_config(5, C.byref(g), C.byref(w), None) == -1
+    assert l.hb_latency_mode(None) == -1
+    h = C.c_void_p()
+    assert l.hb_server_create(0, 0, 0, C.byref(h)) == -1
+    assert b"max_gates" in l.hb_last_error()
+    assert l.hb_server_create(0, (1 << 20) + 1, 0, C.byref(h)) == -1
+    assert l.hb_batch_graph_create(None, 0, 16, None, None, None, None, C.byref(h)) == -1
+    assert b"null" in l.hb_last_error()
+
+
 @pytest.mark.parametrize("set_id", [0, 1, 2])
 def test_encrypt_decrypt_roundtrip(hb, set_id):
     P = hb.Parameters.of(set_id)
