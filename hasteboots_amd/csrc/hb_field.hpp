@@ -121,6 +121,8 @@ struct Bb {
   using word = u32;
   static constexpr u32 P = 0x78000001u;
   static constexpr u32 MU = 0x88000001u;  // P^-1 mod 2^32
+  static constexpr u32 MU_NEG = 0x77FFFFFFu;  // -P^-1 mod 2^32
+  static constexpr u32 R_MOD = 0x0FFFFFFEu;   // 2^32 mod P = 2^28 - 2
   static constexpr u32 MODULUS_BITS = 31;
   static constexpr u64 GENERATOR = 31;
   static constexpr bool IS_GOLDILOCKS = false;

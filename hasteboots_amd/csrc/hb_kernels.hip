@@ -89,14 +89,16 @@ __device__ __forceinline__ void load4(const u32* __restrict__ p, u32 (&v)[4]) {
 }
 
 // acc += NTT(digit) ⊙ key for one gadget row: key rows (a, b) of N slots each. Goldilocks
-// accumulators stay in [0, 2^64) (the inverse transform canonicalises its input).
+// accumulators stay in [0, 2^64) (the inverse transform canonicalises its input); BabyBear
+// accumulators are lazy 64-bit sums of the raw products (one v_mad_u64_u32 each), which
+// gadget_mac folds and reduces.
 // ntt_add_mul_assign_fast (algebra/src/polynomial/univariate/ntt_polynomial.rs:620-629).
 // Key words are software-pipelined one 4-slot group ahead: group 0 arrives in (ka0, kb0),
 // loaded by the caller before the forward transform, and group q+1 is in flight while
 // group q is multiplied; the scheduling barriers keep each load ahead of the arithmetic it
 // overlaps.
-template <class F, int E>
-__device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
+template <class F, int E, class A>
+__device__ __forceinline__ void mac_row(A (&pa)[E], A (&pb)[E],
                                         const typename DF<F>::W (&x)[E],
                                         const typename DF<F>::W* __restrict__ ka,
                                         const typename DF<F>::W* __restrict__ kb,
@@ -120,8 +122,9 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
       if constexpr (F::IS_GOLDILOCKS) {
         gl_mac2(pa[4 * q + r], pb[4 * q + r], x[4 * q + r], va[r], vb[r]);
       } else {
-        pa[4 * q + r] = DF<F>::add(pa[4 * q + r], DF<F>::mul(x[4 * q + r], va[r]));
-        pb[4 * q + r] = DF<F>::add(pb[4 * q + r], DF<F>::mul(x[4 * q + r], vb[r]));
+        static_assert(sizeof(A) == 8, "BabyBear sums are 64-bit");
+        pa[4 * q + r] += (u64)x[4 * q + r] * va[r];
+        pb[4 * q + r] += (u64)x[4 * q + r] * vb[r];
       }
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -134,16 +137,15 @@ __device__ __forceinline__ void mac_row(typename DF<F>::W (&pa)[E], typename DF<
 
 // Σ over one gadget of digit NTTs: for j < len, x = NTT(digit_j(src)), acc += x ⊙ key_j.
 // Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
-// values, digit_j = (v >> j*bits) & mask. tw: forward twiddles in global memory (pass 1),
-// twv: the same table for per-lane reads (LDS copy). S: the source word type (a field word,
-// or u32 for a window of at most 32 bits of digits).
-template <class F, int E, bool PF = true, class S = typename DF<F>::W>
-__device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
-                                           const S (&src)[E], u32 len, u32 bits,
-                                           const typename DF<F>::W* __restrict__ key,
-                                           typename DF<F>::W* lds,
-                                           const typename DF<F>::W* __restrict__ tw,
-                                           const typename DF<F>::W* twv, int lane) {
+// values, digit_j = (v >> j*bits) & mask. twv: the forward twiddles for per-lane reads (LDS
+// copy). S: the source word type (a field word, or u32 for a window of at most 32 bits of
+// digits). A: the accumulator type (field words for Goldilocks, lazy 64-bit sums for
+// BabyBear, folded after every fourth row: 4 (P-1)^2 + 2^60 < 2^64).
+template <class F, int E, bool PF, class S, class A>
+__device__ __forceinline__ void gadget_rows(A (&pa)[E], A (&pb)[E], const S (&src)[E], u32 len,
+                                            u32 bits, const typename DF<F>::W* __restrict__ key,
+                                            typename DF<F>::W* lds,
+                                            const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
   constexpr int N = 64 * E;
   const W mask = (W)((1u << bits) - 1u);
@@ -159,6 +161,31 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
     for (int e = 0; e < E; e++) x[e] = (W)(src[e] >> sh) & mask;
     ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
+    if constexpr (!F::IS_GOLDILOCKS) {
+      if ((j & 3) == 3 && j + 1 < len) {
+#pragma unroll
+        for (int e = 0; e < E; e++) pa[e] = DF<F>::fold(pa[e]), pb[e] = DF<F>::fold(pb[e]);
+      }
+    }
+  }
+}
+
+// gadget_rows on canonical field-word accumulators (pa, pb in and out).
+template <class F, int E, bool PF = true, class S = typename DF<F>::W>
+__device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename DF<F>::W (&pb)[E],
+                                           const S (&src)[E], u32 len, u32 bits,
+                                           const typename DF<F>::W* __restrict__ key,
+                                           typename DF<F>::W* lds,
+                                           const typename DF<F>::W* twv, int lane) {
+  if constexpr (F::IS_GOLDILOCKS) {
+    gadget_rows<F, E, PF>(pa, pb, src, len, bits, key, lds, twv, lane);
+  } else {
+    u64 sa[E], sb[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) sa[e] = DF<F>::lazy_in(pa[e]), sb[e] = DF<F>::lazy_in(pb[e]);
+    gadget_rows<F, E, PF>(sa, sb, src, len, bits, key, lds, twv, lane);
+#pragma unroll
+    for (int e = 0; e < E; e++) pa[e] = DF<F>::lazy_out(sa[e]), pb[e] = DF<F>::lazy_out(sb[e]);
   }
 }
 
@@ -348,7 +375,6 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
   constexpr int E = 16;
   constexpr int N = 64 * E;
   const u32 n = A.n, n1 = A.n + 1;
-  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ ksk = static_cast<const W*>(A.ksk);
   const u32* wi = A.idx ? A.idx + 3 * (size_t)g : nullptr;
   const W b0 = F::add(accB[0], (W)A.q8);
@@ -361,7 +387,7 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
   W pa[E], pb[E];
 #pragma unroll
   for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
-  gadget_mac<F, E, PF>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, tw, twl, lane);
+  gadget_mac<F, E, PF>(pa, pb, src, A.ks_len, A.ks_bits, ksk, lds, twl, lane);
   // b' = coefficient 0 of INTT(init.b + Σ) = b0 + Σ_slots Σ (1/N folded into the KSK).
   W s = 0;
 #pragma unroll
@@ -403,7 +429,6 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   constexpr int N = 64 * E;
   const int lane = threadIdx.x & 63;
   const u32 n = A.n;
-  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
   // Untraced: one accumulator per gate, updated in place. Traced (the vfhe path): an
   // (n+1)-slot history per gate; round i reads slot i and writes slot i+1, so the
@@ -454,7 +479,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
         gate_sync();
         gadget_mac_dig8<E, PF>(pa, pb, dig, A.br_len, ckey, lds, twl, lane);
       } else {
-        gadget_mac<F, E, PF>(pa, pb, dst, A.br_len, A.br_bits, ckey, lds, tw, twl, lane);
+        gadget_mac<F, E, PF>(pa, pb, dst, A.br_len, A.br_bits, ckey, lds, twl, lane);
       }
     }
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
@@ -502,7 +527,6 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int comp = wave / Q, q = wave % Q;
   const u32 n = A.n;
-  const W* __restrict__ tw = static_cast<const W*>(A.tw_fwd);
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
   W* accA = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
   W* accB = accA + N;
@@ -534,13 +558,13 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
         u32 win[E];
 #pragma unroll
         for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
-        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, tw, twl, lane);
+        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, twl, lane);
       } else {
         if (j0 > 0) {
 #pragma unroll
           for (int e = 0; e < E; e++) dst[e] >>= j0 * A.br_bits;
         }
-        gadget_mac<F, E, PF>(pa, pb, dst, j1 - j0, A.br_bits, kj, lds, tw, twl, lane);
+        gadget_mac<F, E, PF>(pa, pb, dst, j1 - j0, A.br_bits, kj, lds, twl, lane);
       }
     }
     if constexpr (WAVES == 8) {
@@ -753,8 +777,6 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
   const size_t idx = (size_t)blockIdx.x * 4 + wave;
   if (idx >= count) return;
   const W* rgsw = static_cast<const W*>(rgsw_);
-  const W* tw = static_cast<const W*>(tw_);
-  const W* itw = static_cast<const W*>(itw_);
   W* lds = lds_all[wave];
   u64* d = rlwe + idx * 2 * N;
   W pa[E], pb[E];
@@ -765,8 +787,7 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
     W src[E];
 #pragma unroll
     for (int e = 0; e < E; e++) src[e] = (W)d[comp * N + lane + 64 * e];
-    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, tw, twl[0],
-                     lane);
+    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, twl[0], lane);
   }
   ntt_inv_wave<F, E>(pa, lds, twl[1], lane);
   ntt_inv_wave<F, E>(pb, lds, twl[1], lane);

@@ -52,18 +52,31 @@ struct DF<Bb> {
   __device__ __forceinline__ static W add(W a, W b) { return Bb::add(a, b); }
   __device__ __forceinline__ static W sub(W a, W b) { return Bb::sub(a, b); }
   // Constants (twiddles, keys) are stored times R = 2^32.
-  // Bb::mont_mul with both products pinned to full 64 bits (the empty asm makes the whole
-  // product live), so they compile to v_mad_u64_u32 rather than v_mul_lo_u32 /
-  // v_mul_hi_u32, which issue slower on gfx950: 4096 BabyBear gates 74.9 -> 67.5 ms.
-  __device__ __forceinline__ static W mul(W a, W c) {
-    const u64 x = (u64)a * c;
-    u64 t = (u64)(u32)x * Bb::MU;
+  // Montgomery reduction of x < 2^32 P, canonical x / 2^32 mod P: with t = -x / P mod 2^32,
+  // x + t P (< 2^33 P < 2^64, one v_mad_u64_u32 with x as the addend) has a zero low word and
+  // its high word is in [0, 2P). t's product is pinned to full 64 bits (the empty asm makes
+  // it live) so it compiles to v_mad_u64_u32 rather than v_mul_lo_u32, which issues slower
+  // on gfx950. Three multiplies and a conditional subtract: 5 instructions.
+  __device__ __forceinline__ static W redc(u64 x) {
+    u64 t = (u64)(u32)x * Bb::MU_NEG;
     asm("" : "+v"(t));
-    u64 u = (u64)(u32)t * Bb::P;
-    asm("" : "+v"(u));
-    const u32 r = (u32)(x >> 32) - (u32)(u >> 32);  // low words cancel: x = u mod 2^32
-    return Bb::umin(r, r + Bb::P);
+    const u32 r = (u32)(((u64)(u32)t * Bb::P + x) >> 32);
+    return Bb::umin(r, r - Bb::P);
   }
+  // The product is pinned too: otherwise the compiler recomputes its low word with a
+  // separate v_mul_lo_u32 and adds the full product in last (four multiplies, not three).
+  __device__ __forceinline__ static W mul(W a, W c) {
+    u64 x = (u64)a * c;
+    asm("" : "+v"(x));
+    return redc(x);
+  }
+  // Lazy sums of products: s = hi 2^32 + lo = hi R_MOD + lo (mod P), below 2^60 for any s.
+  __device__ __forceinline__ static u64 fold(u64 s) {
+    return (u64)(u32)(s >> 32) * Bb::R_MOD + (u32)s;
+  }
+  // A canonical accumulator a entering a lazy sum in Montgomery scale (a 2^32 mod P), < 2^59.
+  __device__ __forceinline__ static u64 lazy_in(W a) { return (u64)a * Bb::R_MOD; }
+  __device__ __forceinline__ static W lazy_out(u64 s) { return redc(fold(s)); }
   __device__ __forceinline__ static W canon(W a) { return a; }
 };
 
