@@ -140,7 +140,8 @@ __device__ __forceinline__ void mac_row(A (&pa)[E], A (&pb)[E],
 // values, digit_j = (v >> j*bits) & mask. twv: the forward twiddles for per-lane reads (LDS
 // copy). S: the source word type (a field word, or u32 for a window of at most 32 bits of
 // digits). A: the accumulator type (field words for Goldilocks, lazy 64-bit sums for
-// BabyBear, folded after every fourth row: 4 (P-1)^2 + 2^60 < 2^64).
+// BabyBear, folded after every second row: the transform's words are below 2P, so
+// 2 (2P-1)(P-1) + 2^60 < 2^64).
 template <class F, int E, bool PF, class S, class A>
 __device__ __forceinline__ void gadget_rows(A (&pa)[E], A (&pb)[E], const S (&src)[E], u32 len,
                                             u32 bits, const typename DF<F>::W* __restrict__ key,
@@ -162,7 +163,7 @@ __device__ __forceinline__ void gadget_rows(A (&pa)[E], A (&pb)[E], const S (&sr
     ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
     if constexpr (!F::IS_GOLDILOCKS) {
-      if ((j & 3) == 3 && j + 1 < len) {
+      if ((j & 1) == 1 && j + 1 < len) {
 #pragma unroll
         for (int e = 0; e < E; e++) pa[e] = DF<F>::fold(pa[e]), pb[e] = DF<F>::fold(pb[e]);
       }

@@ -77,7 +77,8 @@ struct DF<Bb> {
   // A canonical accumulator a entering a lazy sum in Montgomery scale (a 2^32 mod P), < 2^59.
   __device__ __forceinline__ static u64 lazy_in(W a) { return (u64)a * Bb::R_MOD; }
   __device__ __forceinline__ static W lazy_out(u64 s) { return redc(fold(s)); }
-  __device__ __forceinline__ static W canon(W a) { return a; }
+  // Words in [0, 2P) (forward-transform outputs) to canonical.
+  __device__ __forceinline__ static W canon(W a) { return Bb::umin(a, a - Bb::P); }
 };
 
 template <class F>
@@ -121,6 +122,38 @@ struct Bfly {
     gs_bfly<F>(xb, yb, wb);
   }
   // Inverse-transform input normalisation (identity: inputs are canonical).
+  __device__ __forceinline__ static W inv_in(W x) { return x; }
+};
+
+// BabyBear CT butterflies on lazy words in [0, 2P) (2P < 2^32): x is reduced on entry, the
+// product is canonical, and the outputs x + t and x - t + P stay in [0, 2P), one instruction
+// fewer than canonical outputs. A forward transform therefore returns words in [0, 2P)
+// (DF<Bb>::canon reduces them); GS butterflies keep canonical words.
+template <>
+struct Bfly<Bb> {
+  using W = u32;
+  __device__ __forceinline__ static void ct(W& x, W& y, W w) {
+    const W xc = Bb::umin(x, x - Bb::P);
+    const W t = DF<Bb>::mul(y, w);
+    y = xc + Bb::P - t;
+    x = xc + t;
+  }
+  __device__ __forceinline__ static void ct2_u(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    ct(xa, ya, wa);
+    ct(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void ct2_v(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    ct(xa, ya, wa);
+    ct(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void gs2_u(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    gs_bfly<Bb>(xa, ya, wa);
+    gs_bfly<Bb>(xb, yb, wb);
+  }
+  __device__ __forceinline__ static void gs2_v(W& xa, W& ya, W wa, W& xb, W& yb, W wb) {
+    gs_bfly<Bb>(xa, ya, wa);
+    gs_bfly<Bb>(xb, yb, wb);
+  }
   __device__ __forceinline__ static W inv_in(W x) { return x; }
 };
 
@@ -348,7 +381,8 @@ __device__ __forceinline__ int lds_base3(int lane) { return 4 * lane + 4 * (lane
 // Keeps loads issued above it ahead of the arithmetic below it.
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
-// Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r.
+// Forward transform. In: x[e] = position lane + 64e. Out: x[4q+r] = slot 4(lane+64q)+r,
+// words in [0, 2^64) (Goldilocks) or [0, 2P) (BabyBear); DF<F>::canon reduces them.
 // Pass 1 uses the compile-time twiddles (Pass1Tw); twv is psi_rev (psi^bitrev(k) in the
 // field's constant form) for the per-lane reads of passes 2 and 3, normally a copy in
 // LDS. Each stage's twiddles are requested one stage ahead of their use.
@@ -540,10 +574,6 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
   wave_sync();
   // pass 1
   pass1_all<F, E, true, 0>(x, std::make_integer_sequence<int, LOGE>{});
-  if constexpr (!F::IS_GOLDILOCKS) {  // the Goldilocks GS pair already returns canonical words
-#pragma unroll
-    for (int e = 0; e < E; e++) x[e] = DF<F>::canon(x[e]);
-  }
 }
 
 }  // namespace hb
