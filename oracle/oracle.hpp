@@ -46,25 +46,21 @@ struct Gl {
   static constexpr u32 MODULUS_BITS = 64;  // 64 - P.leading_zeros()
   static constexpr u64 GENERATOR = 7;
 
+  // The conditional corrections are written as masks (0 - flag) & EPS so that the
+  // compiler emits no data-dependent branches (a carry is a coin flip for random words).
+  static inline u64 eps_if(bool c) { return (0 - (u64)c) & EPS; }
   // ops.rs:27-34
   static inline u64 add(u64 x, u64 y) {
     u64 sum = x + y;
-    bool over = sum < x;
-    u64 adj = over ? EPS : 0;
-    u64 sum2 = sum + adj;
-    bool over2 = sum2 < sum;
-    if (over2) sum2 += EPS;
-    return sum2;
+    u64 sum2 = sum + eps_if(sum < x);
+    return sum2 + eps_if(sum2 < sum);
   }
   // ops.rs:48-55
   static inline u64 sub(u64 x, u64 y) {
     u64 diff = x - y;
-    bool under = x < y;
-    u64 adj = under ? EPS : 0;
+    u64 adj = eps_if(x < y);
     u64 diff2 = diff - adj;
-    bool under2 = diff < adj;
-    if (under2) diff2 -= EPS;
-    return diff2;
+    return diff2 - eps_if(diff < adj);
   }
   // ops.rs:69-71 — wraps in release builds for a raw x > P. Such events are counted
   // (per thread) so tests can prove a workload exercises them.
@@ -81,13 +77,10 @@ struct Gl {
     u64 x_lo = (u64)x, x_hi = (u64)(x >> 64);
     u64 x_hi_hi = x_hi >> 32;
     u64 x_hi_lo = x_hi & EPS;
-    u64 t0 = x_lo - x_hi_hi;
-    bool borrow = x_lo < x_hi_hi;
-    if (borrow) t0 -= EPS;
+    u64 t0 = x_lo - x_hi_hi - eps_if(x_lo < x_hi_hi);
     u64 t1 = x_hi_lo * EPS;
     u64 res = t0 + t1;
-    bool carry = res < t0;
-    return res + (carry ? EPS : 0);
+    return res + eps_if(res < t0);
   }
   // ops.rs:85-87
   static inline u64 mul(u64 x, u64 y) { return reduce128((u128)x * y); }
