@@ -7,7 +7,9 @@ the timed region starts. One step = one such launch. Also reported in the same r
   roofline — key-stream + I/O bytes per launch over the kernel's event-timed duration vs
              8 TB/s: every gate's wavefront streams the whole BSK + KSK for itself (tiles of
              one gate per key load), so a launch of G gates makes G key-stream passes,
-  cpu_baseline — the CPU oracle (a restatement of the reference path) on the host cores.
+  cpu_baseline — the CPU oracle (a restatement of the reference path) on the host cores,
+  single_nand — one NAND per call for each parameter set on the GPU and on one CPU thread,
+             beside the reference's published single-NAND latencies.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -157,6 +159,46 @@ def mixed_leg(hb, seed: int = 41) -> dict:
             "gates_per_s_end_to_end": round(total / wall, 1), "decrypt_mismatches": int(wrong)}
 
 
+# Table 1 of the reference's performance workbook (statistics/HasteBoots_Performance.xlsx,
+# "Table 2"!B4:H6): one NAND, single thread, Apple M4 Pro (inferred).
+PUBLISHED_SINGLE_NAND_MS = {"zama": 160.18, "babybear": 28.497, "goldilocks": 107.96}
+
+
+def single_nand_leg(hb, cpu: bool, reps: int = 5, seed: int = 61) -> dict:
+    """One NAND per call for every parameter set, the published benchmark's shape: GPU
+    wall time per gate (host ciphertexts in and out, latency-mode launch), and the CPU
+    oracle's single-thread time on the same keys, beside the published figure."""
+    out = {}
+    for name, set_id in SETS.items():
+        P = hb.Parameters.of(set_id)
+        sk = hb.KeyGen.generate_secret_key(P, seed=seed + set_id)
+        bsk, ksk = sk.generate_keys_host()
+        ev = hb.Evaluator(keys=(bsk, ksk), params=P, device=0)
+        enc, dec = hb.Encryptor(sk), hb.Decryptor(sk)
+        rng = np.random.default_rng([seed, set_id])
+        b0, b1 = rng.integers(0, 2, size=(2, reps + 1))
+        c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+        ev.gate_batch("nand", c0[:1], c1[:1])  # warm-up
+        lat, wrong = [], 0
+        for i in range(1, reps + 1):
+            s = time.perf_counter()
+            o = ev.gate_batch("nand", c0[i:i + 1], c1[i:i + 1])
+            lat.append((time.perf_counter() - s) * 1e3)
+            wrong += dec.decrypt(hb.LWECiphertext(o[0])) != 1 - (int(b0[i]) & int(b1[i]))
+        row = {"gpu_p50_ms": round(float(np.percentile(lat, 50)), 3), "wrong": int(wrong),
+               "published_ms": PUBLISHED_SINGLE_NAND_MS[name]}
+        if cpu:
+            import oracle
+            ok = oracle.OracleKeys(set_id, bsk, ksk)
+            s = time.perf_counter()
+            ok.gate_batch(0, c0[:1], c1[:1], threads=1)
+            row["cpu_port_single_thread_ms"] = round((time.perf_counter() - s) * 1e3, 2)
+        out[name] = row
+        log(f"single NAND {name}: {row}")
+    out["published_hardware"] = "Apple M4 Pro (inferred), single thread"
+    return out
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -188,6 +230,8 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager launches instead of replays of the captured hipGraph")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter leg")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the one-NAND-per-call leg of every parameter set")
     ap.add_argument("--no-micro", action="store_true",
                     help="skip the NTT / external-product / MAC-peak microbenchmarks")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -367,6 +411,10 @@ def main() -> int:
         mixed = mixed_leg(hb)
         log(f"mixed batch: {mixed}")
 
+    single = None
+    if not args.no_single:
+        single = single_nand_leg(hb, cpu=not args.no_cpu)
+
     micro = None
     if not args.no_micro:
         micro = microbench(L, check, hb)
@@ -460,6 +508,7 @@ def main() -> int:
                  "mac_peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
                                     "8192 wavefronts"},
         "end_to_end": e2e,
+        "single_nand": single,
         "mixed_batch": mixed,
         "microbench": micro,
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
