@@ -507,14 +507,16 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // c = w / Q of the rotated accumulator (Q = WAVES / 2 waves per component) and the q = w % Q
 // share of its gadget digits, [q l / Q, (q + 1) l / Q), accumulating a partial NTT-domain
 // product. The partials meet in LDS (with 8 waves, waves 4-7 first hand theirs to waves
-// 0-3); wave 0 finishes the a half (sum, inverse NTT, accumulator add) and wave 1 the b
-// half. Wave 0 then key-switches. With 8 waves each SIMD holds two waves of the gate, which
-// hide each other's latency, and a wave's digit window (at most 4 digits) fits 32 bits.
-// Untraced semantics only.
+// 0-3). With 4 waves, wave 0 finishes the a half (sum, inverse NTT, accumulator add) and
+// wave 1 the b half; with 8, waves 0-3 finish the a half and waves 4-7 the b half, a
+// quarter of the slots each (ntt_inv_quarter / ntt_inv_cross, exchanging through the
+// waves' LDS buffers, `ldsw` words apart). Wave 0 then key-switches. With 8 waves each
+// SIMD holds two waves of the gate, which hide each other's latency, and a wave's digit
+// window (at most 4 digits) fits 32 bits. Untraced semantics only.
 // ----------------------------------------------------------------------------
 template <class F, int WAVES>
 __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
-                                                     typename DF<F>::W* lds,
+                                                     typename DF<F>::W* lds, int ldsw,
                                                      unsigned short* ms,
                                                      typename DF<F>::W* part,
                                                      const typename DF<F>::W* twl,
@@ -594,7 +596,29 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       }
     }
     __syncthreads();
-    if (wave < 2) {
+    if constexpr (WAVES == 8) {
+      // waves 0-3 finish the a half, waves 4-7 the b half, a quarter of the slots each:
+      // sum of the four partials, the four-wave inverse transform, accumulator add
+      const int c = wave >> 2, qw = wave & 3;
+      W x[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int e = 4 * qw + k;
+        W v = part[(size_t)c * N + e * 64 + lane];
+#pragma unroll
+        for (int w = 1; w < 4; w++) v = DF<F>::add(v, part[(size_t)w * 2 * N + c * N + e * 64 + lane]);
+        x[k] = v;
+      }
+      ntt_inv_quarter<F>(x, lds, itwl, lane, qw);
+      __syncthreads();
+      ntt_inv_cross<F>(x, lds - (ptrdiff_t)qw * ldsw, ldsw, itwl, lane, qw);
+      W* acc = c ? accB : accA;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int pos = 64 * qw + lane + 256 * k;
+        acc[pos] = F::add(acc[pos], x[k]);
+      }
+    } else if (wave < 2) {
       // wave 0 sums the a partials, wave 1 the b partials
       W* acc = wave ? accB : accA;
       W s[E];
@@ -657,8 +681,8 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   __shared__ W twl[2][1024];
   __shared__ W part[4 * 2 * 1024];
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
-  bootstrap_gate_split<F, WAVES>(A, blockIdx.x, lds[threadIdx.x >> 6], ms, part, twl[0],
-                                 twl[1]);
+  bootstrap_gate_split<F, WAVES>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms, part,
+                                 twl[0], twl[1]);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -703,11 +727,11 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   const int w = threadIdx.x >> 6;
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
-    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], ms, part, twl[0], twl[1]);
+    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl[0], twl[1]);
   } else {
     u32* t = reinterpret_cast<u32*>(&twl[0][0]);
     stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * WAVES);
-    bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), ms,
+    bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64), ms,
                                     reinterpret_cast<u32*>(part), t, t + 1024);
   }
 }

@@ -576,4 +576,86 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
   pass1_all<F, E, true, 0>(x, std::make_integer_sequence<int, LOGE>{});
 }
 
+// Inverse transform (without 1/N) shared by four wavefronts, one quarter each, for the
+// latency-mode kernels: the same Gentleman-Sande butterflies and twiddles as
+// ntt_inv_wave (stage t pairs positions j, j + t with itwv[N/(2t) + j/(2t)]), with each
+// lane holding 4 words instead of 16. Wave qw (0..3) takes slots [256 qw, 256 qw + 256)
+// (x[k] = slot 4(lane + 64 qw) + k, as x[4 qw + k] of ntt_fwd_wave's layout) and runs
+// stages t = 1..128 through its own LDS buffer `lds` (words [0, 316)):
+//   t = 1, 2:     positions 4(lane + 64 qw) + k
+//   t = 4, 8:     256 qw + 16(lane >> 2) + (lane & 3) + 4k
+//   t = 16, 32:   256 qw + 64(lane >> 4) + (lane & 15) + 16k
+//   t = 64, 128:  256 qw + lane + 64k
+// leaving its quarter in `lds` (at coop_a3 of the quarter-local position). After a
+// workgroup barrier ntt_inv_cross reads positions 64 qw + lane + 256k from the four
+// quarters' buffers and runs t = 256, 512; out: x[k] = position 64 qw + lane + 256k,
+// canonical. The layouts are padded so that the 64 lanes' words spread over the banks.
+__device__ __forceinline__ int coop_a1(int p) { return p + (p >> 3); }
+__device__ __forceinline__ int coop_a2(int p) { return p + 4 * (p >> 4); }
+__device__ __forceinline__ int coop_a3(int p) { return p + 4 * (p >> 6); }
+
+template <class F>
+__device__ __forceinline__ void ntt_inv_quarter(typename DF<F>::W (&x)[4], typename DF<F>::W* lds,
+                                                const typename DF<F>::W* itwv, int lane, int qw) {
+  using W = typename DF<F>::W;
+#pragma unroll
+  for (int k = 0; k < 4; k++) x[k] = Bfly<F>::inv_in(x[k]);
+  {  // t = 1, 2
+    const int c = lane + 64 * qw;
+    const W wa = itwv[512 + 2 * c], wb = itwv[512 + 2 * c + 1], w2 = itwv[256 + c];
+    Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
+    Bfly<F>::gs2_v(x[0], x[2], w2, x[1], x[3], w2);
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[coop_a1(4 * lane + k)] = x[k];
+  }
+  wave_sync();
+  {  // t = 4, 8
+    const int g = 16 * qw + (lane >> 2), base = 16 * (lane >> 2) + (lane & 3);
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = lds[coop_a1(base + 4 * k)];
+    const W wa = itwv[128 + 2 * g], wb = itwv[128 + 2 * g + 1], w8 = itwv[64 + g];
+    Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
+    Bfly<F>::gs2_v(x[0], x[2], w8, x[1], x[3], w8);
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[coop_a2(base + 4 * k)] = x[k];
+  }
+  wave_sync();
+  {  // t = 16, 32
+    const int b = 4 * qw + (lane >> 4), base = 64 * (lane >> 4) + (lane & 15);
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = lds[coop_a2(base + 16 * k)];
+    const W wa = itwv[32 + 2 * b], wb = itwv[32 + 2 * b + 1], w32 = itwv[16 + b];
+    Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
+    Bfly<F>::gs2_v(x[0], x[2], w32, x[1], x[3], w32);
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[coop_a3(base + 16 * k)] = x[k];
+  }
+  wave_sync();
+  {  // t = 64, 128
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = lds[coop_a3(lane + 64 * k)];
+    const W wa = itwv[8 + 2 * qw], wb = itwv[8 + 2 * qw + 1], w128 = itwv[4 + qw];
+    Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
+    Bfly<F>::gs2_v(x[0], x[2], w128, x[1], x[3], w128);
+#pragma unroll
+    for (int k = 0; k < 4; k++) lds[coop_a3(lane + 64 * k)] = x[k];
+  }
+}
+
+// Stages t = 256, 512 of the four-wave inverse transform; q0: the LDS buffer of the wave
+// holding quarter 0, quarter k's buffer at q0 + k * stride (in words). Call after a
+// barrier that follows every quarter's ntt_inv_quarter.
+template <class F>
+__device__ __forceinline__ void ntt_inv_cross(typename DF<F>::W (&x)[4],
+                                              const typename DF<F>::W* q0, int stride,
+                                              const typename DF<F>::W* itwv, int lane, int qw) {
+  using W = typename DF<F>::W;
+  const int a = coop_a3(64 * qw + lane);
+#pragma unroll
+  for (int k = 0; k < 4; k++) x[k] = q0[(size_t)k * stride + a];
+  const W w2 = itwv[2], w3 = itwv[3], w1 = itwv[1];
+  Bfly<F>::gs2_v(x[0], x[1], w2, x[2], x[3], w3);
+  Bfly<F>::gs2_v(x[0], x[2], w1, x[1], x[3], w1);
+}
+
 }  // namespace hb
