@@ -160,8 +160,9 @@ int hb_tuned_config(int set, int* gates_per_workgroup, int* waves_per_simd, int*
 /* Pins the variant for a parameter set. */
 int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
 /* Latency mode: batches (hb_nand, hb_gate*, hb_bootstrap, hb_gate_batch*, captured
- * graphs) of at most max_gates gates run 4 wavefronts per gate, each wave taking a
- * quarter of the digit transforms, which cuts a gate's latency about 3x at lower peak
+ * graphs) of at most max_gates gates run 8 wavefronts per gate (4 for BabyBear launches of
+ * more than one gate per compute unit), each wave taking a share of the digit transforms
+ * and of the inverse transforms, which cuts a gate's latency about 4x at lower peak
  * throughput. Default: three gates per compute unit (768 on MI355X, where latency mode
  * stays faster than one wavefront per gate); 0 disables. */
 int hb_set_latency_mode(uint32_t max_gates);
