@@ -526,7 +526,10 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   constexpr int E = 16;
   constexpr int N = 64 * E;
   constexpr int Q = WAVES / 2;
-  constexpr bool PF = WAVES == 4;  // two waves per SIMD: twiddles read just before use
+  // Twiddles requested a stage ahead even with two waves per SIMD: the waves of one gate run
+  // in step, so they hide little of each other's LDS latency (256 Goldilocks gates with 8
+  // waves: 18.8 -> 17.9 ms).
+  constexpr bool PF = true;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int comp = wave / Q, q = wave % Q;
   const u32 n = A.n;
