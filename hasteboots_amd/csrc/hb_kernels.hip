@@ -507,10 +507,10 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // c = w / Q of the rotated accumulator (Q = WAVES / 2 waves per component) and the q = w % Q
 // share of its gadget digits, [q l / Q, (q + 1) l / Q), accumulating a partial NTT-domain
 // product. The partials meet in LDS (with 8 waves, waves 4-7 first hand theirs to waves
-// 0-3). With 4 waves, wave 0 finishes the a half (sum, inverse NTT, accumulator add) and
-// wave 1 the b half; with 8, waves 0-3 finish the a half and waves 4-7 the b half, a
-// quarter of the slots each (ntt_inv_quarter / ntt_inv_cross, exchanging through the
-// waves' LDS buffers, `ldsw` words apart). Wave 0 then key-switches. With 8 waves each
+// 0-3). Then every wave finishes quarters of the result (sum of the partials, inverse
+// NTT, accumulator add) with ntt_inv_quarter / ntt_inv_cross, exchanging through the
+// waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
+// each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then key-switches. With 8 waves each
 // SIMD holds two waves of the gate, which hide each other's latency, and a wave's digit
 // window (at most 4 digits) fits 32 bits. Untraced semantics only.
 // ----------------------------------------------------------------------------
@@ -621,22 +621,32 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
         const int pos = 64 * qw + lane + 256 * k;
         acc[pos] = F::add(acc[pos], x[k]);
       }
-    } else if (wave < 2) {
-      // wave 0 sums the a partials, wave 1 the b partials
-      W* acc = wave ? accB : accA;
-      W s[E];
+    } else {
+      // wave w finishes quarter w of both halves: the a quarter in words [0, 320) of its
+      // LDS buffer, the b quarter in [320, 640)
+      const int qw = wave;
+      W xa[4], xb[4];
 #pragma unroll
-      for (int e = 0; e < E; e++) {
-        W v = part[(size_t)0 * 2 * N + wave * N + e * 64 + lane];
+      for (int k = 0; k < 4; k++) {
+        const int e = 4 * qw + k;
+        W va = part[(size_t)e * 64 + lane], vb = part[(size_t)N + e * 64 + lane];
 #pragma unroll
-        for (int w = 1; w < 4; w++) v = DF<F>::add(v, part[(size_t)w * 2 * N + wave * N + e * 64 + lane]);
-        s[e] = v;
+        for (int w = 1; w < 4; w++) {
+          va = DF<F>::add(va, part[(size_t)w * 2 * N + e * 64 + lane]);
+          vb = DF<F>::add(vb, part[(size_t)w * 2 * N + N + e * 64 + lane]);
+        }
+        xa[k] = va, xb[k] = vb;
       }
-      ntt_inv_wave<F, E, PF>(s, lds, itwl, lane);
+      ntt_inv_quarter<F>(xa, lds, itwl, lane, qw);
+      ntt_inv_quarter<F>(xb, lds + 320, itwl, lane, qw);
+      __syncthreads();
+      ntt_inv_cross<F>(xa, lds - (ptrdiff_t)qw * ldsw, ldsw, itwl, lane, qw);
+      ntt_inv_cross<F>(xb, lds - (ptrdiff_t)qw * ldsw + 320, ldsw, itwl, lane, qw);
 #pragma unroll
-      for (int e = 0; e < E; e++) {
-        const int k = lane + 64 * e;
-        acc[k] = F::add(acc[k], s[e]);
+      for (int k = 0; k < 4; k++) {
+        const int pos = 64 * qw + lane + 256 * k;
+        accA[pos] = F::add(accA[pos], xa[k]);
+        accB[pos] = F::add(accB[pos], xb[k]);
       }
     }
     __syncthreads();
