@@ -55,19 +55,20 @@ struct Engine {
   // 1-gate-per-workgroup, 1-wave-per-SIMD variant: spreading them over every SIMD beats
   // doubling up waves on half of them.
   uint32_t simd_slots = 1024;
-  // Batches of at most this many untraced gates run in latency mode (4 wavefronts per
-  // gate, one workgroup per gate, one such workgroup resident per CU); by default three
-  // per CU. Measured on MI355X (256 CUs), Goldilocks: 256 gates 21 ms, 257-512 42 ms,
-  // 513-768 64 ms in latency mode against 67-71 ms with one wavefront per gate, which
-  // wins from 1024 gates (72 against 85 ms); the other parameter sets cross over alike.
+  // Batches of at most this many untraced gates run in latency mode (split_waves
+  // wavefronts per gate, one workgroup per gate); by default three per CU. Measured on
+  // MI355X (256 CUs), Goldilocks: 256 gates 17.9 ms, 512 35.8 ms, 768 53.8 ms in latency
+  // mode against 67-71 ms with one wavefront per gate, which ties at 1024 gates (72.0
+  // against 71.6 ms); Zama loses there (104 against 113 ms), BabyBear still wins (16.4
+  // against 19.5 ms).
   uint32_t split_max = 768;
 };
 
 // Wavefronts per gate in latency mode: 8 (two per SIMD, hiding each other's latency) for
 // the Goldilocks sets, and for any launch of at most one gate per CU; otherwise 4, for
 // BabyBear, whose 4-wave workgroups fit two to a CU. Measured on MI355X (kernel ms, 8 / 4
-// waves): Goldilocks 256 gates 19.6 / 21.3, 512 39.3 / 42.4; Zama 256 30.5 / 31.9;
-// BabyBear 256 8.6 / 9.9, 512 17.2 / 13.1. With 8 waves a wave's share of a gadget (at
+// waves): Goldilocks 256 gates 17.9 / 19.5, 512 35.8 / 39.0; Zama 256 28.2 / 30.2;
+// BabyBear 256 5.9 / 6.0, 512 11.5 / 8.5, 768 17.2 / 13.1. With 8 waves a wave's share of a gadget (at
 // most ceil(len / 4) digits) is held in 32 bits, which every parameter set here meets
 // (`narrow` false falls back to 4 waves).
 inline bool split8_fits(uint32_t br_len, uint32_t br_bits) {
