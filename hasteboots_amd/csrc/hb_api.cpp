@@ -66,7 +66,14 @@ const int kVariants[HB_TUNE_VARIANTS][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 
 // Variant used for a batch of `count` gates of parameter set `set`; gpw = 0 selects the
 // latency-mode kernel (split_waves wavefronts per gate).
 void batch_variant(const Engine& E, int set, size_t count, int* gpw, int* wps) {
-  if (count <= E.split_max) {
+  // Unless hb_set_latency_mode chose one cutoff for every set, BabyBear keeps latency mode
+  // up to one gate per SIMD: its 4-wave workgroups (two per CU) stay ahead of one wavefront
+  // per gate that far (1024 gates: 16.4 against 19.5 ms on MI355X), the Goldilocks sets'
+  // only up to three gates per CU.
+  const uint32_t cutoff = set == SET_BABYBEAR && !E.split_user
+                              ? std::max(E.split_max, E.simd_slots)
+                              : E.split_max;
+  if (count <= cutoff) {
     *gpw = 0, *wps = 1;
   } else if (count <= E.simd_slots) {
     *gpw = 1, *wps = 1;
@@ -1010,6 +1017,7 @@ int hb_set_latency_mode(uint32_t max_gates) {
   int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
   if (rc) return rc;
   engine().split_max = max_gates;
+  engine().split_user = true;
   return HB_OK;
 }
 

@@ -62,6 +62,9 @@ struct Engine {
   // against 71.6 ms); Zama loses there (104 against 113 ms), BabyBear still wins (16.4
   // against 19.5 ms).
   uint32_t split_max = 768;
+  // Set by hb_set_latency_mode: split_max then applies to every parameter set (by default
+  // BabyBear's cutoff is one gate per SIMD, see batch_variant).
+  bool split_user = false;
 };
 
 // Wavefronts per gate in latency mode: 8 (two per SIMD, hiding each other's latency) for

@@ -164,7 +164,9 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * more than one gate per compute unit), each wave taking a share of the digit transforms
  * and of the inverse transforms, which cuts a gate's latency about 4x at lower peak
  * throughput. Default: three gates per compute unit (768 on MI355X, where latency mode
- * stays faster than one wavefront per gate); 0 disables. */
+ * stays faster than one wavefront per gate), and one per SIMD (1024) for BabyBear, whose
+ * latency mode stays ahead that far. A call sets one cutoff for every parameter set;
+ * 0 disables. hb_latency_mode reports the cutoff of the Goldilocks sets. */
 int hb_set_latency_mode(uint32_t max_gates);
 int hb_latency_mode(uint32_t* max_gates);
 
