@@ -510,8 +510,8 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // 0-3). Then every wave finishes quarters of the result (sum of the partials, inverse
 // NTT, accumulator add) with ntt_inv_quarter / ntt_inv_cross, exchanging through the
 // waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
-// each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then key-switches. With 8 waves each
-// SIMD holds two waves of the gate, which hide each other's latency, and a wave's digit
+// each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then
+// key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
 // window (at most 4 digits) fits 32 bits. Untraced semantics only.
 // ----------------------------------------------------------------------------
 template <class F, int WAVES>
