@@ -119,45 +119,67 @@ static const NttTable<F>& table(u32 logN) {
   return *it->second;
 }
 
+// One stage of butterflies of distance t over m blocks, block i with twiddle S[i]. The
+// stages with t < 8 are instantiated with a compile-time distance so that the block body
+// unrolls and the compiler can vectorise across blocks (same arithmetic, same words).
+template <class F, u32 TT>
+static inline void ct_stage(typename F::word* a, const typename F::word* S, u32 m, u32 t) {
+  using W = typename F::word;
+  const u32 tt = TT ? TT : t;
+  for (u32 i = 0; i < m; i++) {
+    const W s = S[i];
+    W* x = a + 2 * i * tt;
+    W* y = x + tt;
+    for (u32 j = 0; j < tt; j++) {
+      W U = x[j];
+      W V = F::mul(y[j], s);
+      x[j] = F::add(U, V);
+      y[j] = F::sub(U, V);
+    }
+  }
+}
+template <class F, u32 TT>
+static inline void gs_stage(typename F::word* a, const typename F::word* S, u32 h, u32 t) {
+  using W = typename F::word;
+  const u32 tt = TT ? TT : t;
+  for (u32 i = 0; i < h; i++) {
+    const W s = S[i];
+    W* x = a + 2 * i * tt;
+    W* y = x + tt;
+    for (u32 j = 0; j < tt; j++) {
+      W U = x[j], V = y[j];
+      x[j] = F::add(U, V);
+      y[j] = F::mul(F::sub(U, V), s);
+    }
+  }
+}
+
 template <class F>
 static void ntt_fwd(typename F::word* a, const NttTable<F>& T) {
-  using W = typename F::word;
   const u32 N = T.N;
   u32 t = N;
   for (u32 m = 1; m < N; m <<= 1) {
     t >>= 1;
-    for (u32 i = 0; i < m; i++) {
-      const W S = T.psi_rev[m + i];
-      W* x = a + 2 * i * t;
-      W* y = x + t;
-      for (u32 j = 0; j < t; j++) {
-        W U = x[j];
-        W V = F::mul(y[j], S);
-        x[j] = F::add(U, V);
-        y[j] = F::sub(U, V);
-      }
-    }
+    const auto* S = T.psi_rev.data() + m;
+    if (t == 1) ct_stage<F, 1>(a, S, m, t);
+    else if (t == 2) ct_stage<F, 2>(a, S, m, t);
+    else if (t == 4) ct_stage<F, 4>(a, S, m, t);
+    else ct_stage<F, 0>(a, S, m, t);
   }
   for (u32 j = 0; j < N; j++) a[j] = F::canon(a[j]);
 }
 
 template <class F>
 static void ntt_inv(typename F::word* a, const NttTable<F>& T) {
-  using W = typename F::word;
   const u32 N = T.N;
   u32 t = 1;
   for (u32 m = N; m > 1; m >>= 1) {
     u32 h = m >> 1;
-    for (u32 i = 0; i < h; i++) {
-      const W S = T.psi_inv_rev[h + i];
-      W* x = a + 2 * i * t;
-      W* y = x + t;
-      for (u32 j = 0; j < t; j++) {
-        W U = x[j], V = y[j];
-        x[j] = F::add(U, V);
-        y[j] = F::mul(F::sub(U, V), S);
-      }
-    }
+    const auto* S = T.psi_inv_rev.data() + h;
+    if (t == 1) gs_stage<F, 1>(a, S, h, t);
+    else if (t == 2) gs_stage<F, 2>(a, S, h, t);
+    else if (t == 4) gs_stage<F, 4>(a, S, h, t);
+    else gs_stage<F, 0>(a, S, h, t);
     t <<= 1;
   }
   for (u32 j = 0; j < N; j++) a[j] = F::canon(F::mul(a[j], T.n_inv));
