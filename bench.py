@@ -234,6 +234,10 @@ def main() -> int:
                     help="skip the one-NAND-per-call leg of every parameter set")
     ap.add_argument("--no-micro", action="store_true",
                     help="skip the NTT / external-product / MAC-peak microbenchmarks")
+    ap.add_argument("--only", choices=["micro", "launches"], default=None,
+                    help="run one leg alone and print its JSON: 'micro' the microbenchmarks, "
+                         "'launches' --steps eager launches of --batch gates with the default "
+                         "launch rules (the kernel a rocprofv3 pass of that size measures)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the batched leg, copy a device buffer of this size once "
                          "(a reference kernel for scaling HBM byte counters)")
@@ -253,6 +257,20 @@ def main() -> int:
         log("no GPU visible")
         return 1
     L = lib()
+    if args.copy_probe_mib > 0:
+        import torch
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nb} bytes device to device")
+        del src, dst
+    if args.only == "micro":
+        micro = microbench(L, check, hb)
+        print(json.dumps({"leg": "micro", "microbench": micro}), flush=True)
+        return 0
     set_id = SETS[args.set]
     P = hb.Parameters.of(set_id)
     G, n1 = args.batch, P.n + 1
@@ -276,22 +294,31 @@ def main() -> int:
         check(L.hb_memcpy_h2d(p, arr.ctypes.data, arr.nbytes))
         return p
 
-    if args.copy_probe_mib > 0:
-        import torch
-        nb = args.copy_probe_mib << 20
-        src = torch.ones(nb, dtype=torch.uint8, device="cuda:0")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: {nb} bytes device to device")
-        del src, dst
-
     d0, d1 = dev(c0), dev(c1)
     dout = dev(np.zeros_like(c0))
 
     def launch(count, a=d0, b=d1, o=dout):
         check(L.hb_gate_batch_dev(ev._h, 0, count, a, b, None, o, None))
+
+    if args.only == "launches":
+        launch(G)  # warm-up
+        check(L.hb_sync())
+        kms = []
+        for _ in range(args.steps):
+            launch(G)
+            kms.append(hb.last_kernel_ms())
+        out = np.zeros_like(c0)
+        check(L.hb_memcpy_d2h(out.ctypes.data, dout, out.nbytes))
+        dec = hb.Decryptor(sk)
+        wrong = sum(dec.decrypt(hb.LWECiphertext(out[i])) != 1 - (int(b0[i]) & int(b1[i]))
+                    for i in range(G))
+        print(json.dumps({"leg": "launches", "parameter_set": SET_NAMES[set_id], "gates": G,
+                          "latency_mode_max_gates": hb.latency_mode(),
+                          "kernel_ms": [round(x, 3) for x in kms],
+                          "decrypt_mismatches": int(wrong)}), flush=True)
+        for p in (d0, d1, dout):
+            L.hb_dev_free(p)
+        return 0
 
     tune = None
     if not args.no_autotune:
@@ -443,8 +470,8 @@ def main() -> int:
         ok = oracle.OracleKeys(set_id, bsk, ksk)
         s = time.perf_counter()
         ok.gate_batch(0, c0[:1], c1[:1], threads=1)
-        single = time.perf_counter() - s
-        per_thread = max(1, int(round(args.cpu_seconds / max(single, 1e-3))))
+        cpu_single_s = time.perf_counter() - s
+        per_thread = max(1, int(round(args.cpu_seconds / max(cpu_single_s, 1e-3))))
         per_thread = min(per_thread, max(1, G // threads))
         S = threads * per_thread
         s = time.perf_counter()
@@ -453,11 +480,11 @@ def main() -> int:
         cpu = {"value": round(S / el, 3), "unit": "bootstraps/s", "cores": threads,
                "kind": "port",
                "sample": f"{S} NAND gates ({args.set}) on {threads} threads, one gate per "
-                         f"thread task; single-thread latency {single * 1e3:.1f} ms",
-               "single_thread_ms": round(single * 1e3, 2), "cpu_model": cpu_model(),
+                         f"thread task; single-thread latency {cpu_single_s * 1e3:.1f} ms",
+               "single_thread_ms": round(cpu_single_s * 1e3, 2), "cpu_model": cpu_model(),
                "nproc": os.cpu_count()}
         log(f"cpu baseline: {cpu['value']} bootstraps/s on {threads} threads, "
-            f"single {single * 1e3:.0f} ms")
+            f"single {cpu_single_s * 1e3:.0f} ms")
 
     line = {
         "metric": METRIC,

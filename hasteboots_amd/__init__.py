@@ -3,6 +3,7 @@
 The Python layer mirrors the reference `tfhe` crate's gate API; see tfhe.py.
 """
 from .tfhe import (  # noqa: F401
+    GATES,
     BABYBEAR_BINARY_128_BITS_PARAMETERS,
     GOLDILOCKS_BINARY_128_BITS_PARAMETERS,
     ZAMA_BINARY_128_BITS_PARAMETERS,
@@ -19,8 +20,11 @@ from .tfhe import (  # noqa: F401
     SecretKeyPack,
     device_count,
     last_kernel_ms,
+    LATENCY_DEFAULT,
     latency_mode,
+    latency_modes,
     mixed_batch,
+    restore_latency_modes,
     set_latency_mode,
     set_tuned_config,
     tuned_config,

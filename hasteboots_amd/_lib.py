@@ -55,6 +55,7 @@ class HbTuneResult(C.Structure):
 
 
 TUNE_VARIANTS = [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2), (4, 3)]
+LATENCY_DEFAULT = 0xFFFFFFFF  # HB_LATENCY_DEFAULT
 
 
 class HbError(RuntimeError):
@@ -115,7 +116,9 @@ _PROTOS = {
     "hb_external_product_dev": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "hb_set_latency_mode": (C.c_int, [C.c_uint32]),
+    "hb_set_latency_mode_for": (C.c_int, [C.c_int, C.c_uint32]),
     "hb_latency_mode": (C.c_int, [C.POINTER(C.c_uint32)]),
+    "hb_latency_mode_for": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "hb_modmul_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
     "hb_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -124,6 +127,9 @@ _PROTOS = {
     "hb_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "hb_sync": (C.c_int, []),
     "hb_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "hb_stream_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "hb_stream_destroy": (C.c_int, [C.c_void_p]),
+    "hb_stream_sync": (C.c_int, [C.c_void_p]),
     "hb_last_kernel_ms": (C.c_int, [C.POINTER(C.c_double)]),
     "hb_last_error": (C.c_char_p, []),
 }

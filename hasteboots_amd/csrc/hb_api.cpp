@@ -41,6 +41,28 @@ Engine& engine() {
   return e;
 }
 
+LaunchTimer& launch_timer() {
+  thread_local LaunchTimer t;
+  return t;
+}
+
+int timer_start(hipStream_t s) {
+  LaunchTimer& t = launch_timer();
+  if (!t.ev0) {
+    HIP_TRY(hipEventCreate(&t.ev0));
+    HIP_TRY(hipEventCreate(&t.ev1));
+  }
+  HIP_TRY(hipEventRecord(t.ev0, s));
+  return HB_OK;
+}
+
+int timer_stop(hipStream_t s) {
+  LaunchTimer& t = launch_timer();
+  HIP_TRY(hipEventRecord(t.ev1, s));
+  t.used = true;
+  return HB_OK;
+}
+
 }  // namespace hb
 
 namespace {
@@ -66,14 +88,7 @@ const int kVariants[HB_TUNE_VARIANTS][2] = {{1, 1}, {2, 1}, {4, 1}, {2, 2}, {4, 
 // Variant used for a batch of `count` gates of parameter set `set`; gpw = 0 selects the
 // latency-mode kernel (split_waves wavefronts per gate).
 void batch_variant(const Engine& E, int set, size_t count, int* gpw, int* wps) {
-  // Unless hb_set_latency_mode chose one cutoff for every set, BabyBear keeps latency mode
-  // up to one gate per SIMD: its 4-wave workgroups (two per CU) stay ahead of one wavefront
-  // per gate that far (1024 gates: 16.4 against 19.5 ms on MI355X), the Goldilocks sets'
-  // only up to three gates per CU.
-  const uint32_t cutoff = set == SET_BABYBEAR && !E.split_user
-                              ? std::max(E.split_max, E.simd_slots)
-                              : E.split_max;
-  if (count <= cutoff) {
+  if (count <= E.split_max[set]) {
     *gpw = 0, *wps = 1;
   } else if (count <= E.simd_slots) {
     *gpw = 1, *wps = 1;
@@ -128,8 +143,7 @@ int hb::ensure_engine(int device) {
   if (device < 0 || device >= n) return fail(HB_ERR_ARG, "device %d out of range (%d)", device, n);
   HIP_TRY(hipSetDevice(device));
   HIP_TRY(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
-  HIP_TRY(hipEventCreate(&E.ev0));
-  HIP_TRY(hipEventCreate(&E.ev1));
+  HIP_TRY(hipEventCreateWithFlags(&E.mixed_ev, hipEventDisableTiming));
   // Twiddle tables.
   for (int f = 0; f < 2; f++) {
     ParamSet p;
@@ -162,7 +176,7 @@ int hb::ensure_engine(int device) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
-    E.split_max = 3 * (E.simd_slots / 4);
+    for (int set = 0; set < SET_COUNT; set++) E.split_max[set] = default_split_max(set, E.simd_slots);
   }
   load_tune_cache(E);
   return HB_OK;
@@ -207,11 +221,25 @@ int upload_keys(hb_ek* ek) {
   rc = convert(ek->ksk_host, &ek->ksk_dev);
   if (rc) return rc;
   HIP_TRY(hipMalloc(&ek->lut_dev, (size_t)p.N * 8));
+  HIP_TRY(hipEventCreateWithFlags(&ek->acc_ev, hipEventDisableTiming));
+  return HB_OK;
+}
+
+// The next launch on `s` that uses ek->acc_dev starts after every earlier one (on any
+// stream) has finished with it.
+int acc_acquire(hb_ek* ek, hipStream_t s) {
+  if (ek->acc_ev_used) HIP_TRY(hipStreamWaitEvent(s, ek->acc_ev, 0));
+  return HB_OK;
+}
+int acc_release(hb_ek* ek, hipStream_t s) {
+  HIP_TRY(hipEventRecord(ek->acc_ev, s));
+  ek->acc_ev_used = true;
   return HB_OK;
 }
 
 int ensure_acc(hb_ek* ek, size_t count) {
   if (count <= ek->acc_cap) return HB_OK;
+  if (ek->acc_ev_used) HIP_TRY(hipEventSynchronize(ek->acc_ev));
   if (ek->acc_dev) HIP_TRY(hipFree(ek->acc_dev));
   ek->acc_dev = nullptr;
   ek->acc_cap = 0;
@@ -294,7 +322,10 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   int rc = ensure_acc(ek, count);
   if (rc) return rc;
   BootArgs a = make_args(ek, gate, count, c0, c1, c2, lut_dev, out, ek->acc_dev);
-  HIP_TRY(hipEventRecord(E.ev0, s));
+  rc = acc_acquire(ek, s);
+  if (rc) return rc;
+  rc = timer_start(s);
+  if (rc) return rc;
   int gpw, wps;
   batch_variant(E, p.set, count, &gpw, &wps);
   if (gpw == 0)
@@ -303,17 +334,55 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
         split_waves(E, p.is_goldilocks, count, split8_fits(p.br_len, p.br_bits)), s));
   else
     HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
-  HIP_TRY(hipEventRecord(E.ev1, s));
+  rc = timer_stop(s);
+  if (rc) return rc;
+  return acc_release(ek, s);
+}
+
+int ensure_mux(hb_ek* ek, size_t count) {
+  if (count <= ek->mux_cap) return HB_OK;
+  if (ek->acc_ev_used) HIP_TRY(hipEventSynchronize(ek->acc_ev));
+  if (ek->mux_dev) HIP_TRY(hipFree(ek->mux_dev));
+  ek->mux_dev = nullptr;
+  ek->mux_cap = 0;
+  HIP_TRY(hipMalloc((void**)&ek->mux_dev, count * 6 * (ek->p.n + 1) * sizeof(uint64_t)));
+  ek->mux_cap = count;
   return HB_OK;
 }
 
+// Evaluator::mux (tfhe/src/bfhe/evaluate.rs:245-263) for `count` gates on stream s, select
+// c0, then c1, else c2. Rows of n+1 words in the key's mux staging W: [0, c) c0, [c, 2c)
+// not(c0), [2c, 3c) c1, [3c, 4c) c2; one launch of 2c AND gates (c0 & c1, not(c0) & c2) into
+// [4c, 6c); then one launch of c OR-LUT bootstraps of the pairwise sums into outs (the OR
+// gate's combination is the same component-wise add).
+int launch_mux(hb_ek* ek, size_t count, const uint64_t* c0, const uint64_t* c1,
+               const uint64_t* c2, uint64_t* outs, hipStream_t s) {
+  int rc = ensure_mux(ek, count);
+  if (rc) return rc;
+  rc = acc_acquire(ek, s);  // the previous user of W may run on another stream
+  if (rc) return rc;
+  const ParamSet& p = ek->p;
+  const size_t n1 = p.n + 1, rows = count * n1;
+  uint64_t* W = ek->mux_dev;
+  HIP_TRY(hipMemcpyAsync(W, c0, rows * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(launch_lwe_not(c0, W + rows, (uint32_t)count, (uint32_t)n1, p.q, p.is_goldilocks, s));
+  HIP_TRY(hipMemcpyAsync(W + 2 * rows, c1, rows * 8, hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipMemcpyAsync(W + 3 * rows, c2, rows * 8, hipMemcpyDeviceToDevice, s));
+  rc = launch_gates(ek, GATE_AND, 2 * count, W, W + 2 * rows, nullptr, nullptr, W + 4 * rows, s);
+  if (rc) return rc;
+  return launch_gates(ek, GATE_OR, count, W + 4 * rows, W + 5 * rows, nullptr, nullptr, outs, s);
+}
+
+// Gate kinds the batch entry points take: the seven single-bootstrap gates and mux.
+bool batch_gate_ok(int gate) { return (gate >= 0 && gate <= GATE_MAJORITY) || gate == GATE_MUX; }
+bool needs_c2(int gate) { return gate == GATE_MAJORITY || gate == GATE_MUX; }
+
+// Waits for the calling thread's last timed launch (the host-buffer calls return only
+// after their results are on the host).
 int record_ms(hipStream_t s) {
   (void)s;
-  Engine& E = engine();
-  HIP_TRY(hipEventSynchronize(E.ev1));
-  float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
-  E.last_ms = ms;
+  LaunchTimer& t = launch_timer();
+  HIP_TRY(hipEventSynchronize(t.ev1));
   return HB_OK;
 }
 
@@ -447,7 +516,10 @@ int hb_ek_destroy(hb_ek_t* ek) {
   if (ek->ksk_dev) (void)hipFree(ek->ksk_dev);
   if (ek->lut_dev) (void)hipFree(ek->lut_dev);
   if (ek->io_dev) (void)hipFree(ek->io_dev);
+  if (ek->acc_ev_used) (void)hipEventSynchronize(ek->acc_ev);
   if (ek->acc_dev) (void)hipFree(ek->acc_dev);
+  if (ek->mux_dev) (void)hipFree(ek->mux_dev);
+  if (ek->acc_ev) (void)hipEventDestroy(ek->acc_ev);
   delete ek;
   return HB_OK;
 }
@@ -460,22 +532,24 @@ int hb_ek_params(const hb_ek_t* ek, hb_params_t* out) {
 int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
                       const uint64_t* c2, uint64_t* outs, void* stream) {
   if (!ek || !c0 || !outs) return fail(HB_ERR_ARG, "null argument");
-  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (!batch_gate_ok(gate)) return fail(HB_ERR_ARG, "bad gate %d", gate);
   if (!c1) return fail(HB_ERR_ARG, "gate needs two inputs");
-  if (gate == GATE_MAJORITY && !c2) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (needs_c2(gate) && !c2) return fail(HB_ERR_ARG, "gate %d needs three inputs", gate);
   if (count == 0) return HB_OK;
-  if (count > 0x7FFFFFFF) return fail(HB_ERR_ARG, "count too large");
+  if (count > 0x3FFFFFFF) return fail(HB_ERR_ARG, "count too large");
   hipStream_t s = stream ? (hipStream_t)stream : engine().stream;
   std::lock_guard<std::mutex> g(ek->mu);
+  if (gate == GATE_MUX) return launch_mux(ek, count, c0, c1, c2, outs, s);
   return launch_gates(ek, gate, count, c0, c1, c2, nullptr, outs, s);
 }
 
 int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,
                   const uint64_t* c2s, uint64_t* outs) {
   if (!ek || !c0s || !c1s || !outs) return fail(HB_ERR_ARG, "null argument");
-  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
-  if (gate == GATE_MAJORITY && !c2s) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (!batch_gate_ok(gate)) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (needs_c2(gate) && !c2s) return fail(HB_ERR_ARG, "gate %d needs three inputs", gate);
   if (count == 0) return HB_OK;
+  if (count > 0x3FFFFFFF) return fail(HB_ERR_ARG, "count too large");
   Engine& E = engine();
   std::lock_guard<std::mutex> g(ek->mu);
   int rc = ensure_io(ek, count);
@@ -488,7 +562,9 @@ int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, cons
   HIP_TRY(hipMemcpyAsync(d0, c0s, bytes, hipMemcpyHostToDevice, E.stream));
   HIP_TRY(hipMemcpyAsync(d1, c1s, bytes, hipMemcpyHostToDevice, E.stream));
   if (c2s) HIP_TRY(hipMemcpyAsync(d2, c2s, bytes, hipMemcpyHostToDevice, E.stream));
-  rc = launch_gates(ek, gate, count, d0, d1, c2s ? d2 : nullptr, nullptr, dout, E.stream);
+  rc = gate == GATE_MUX ? launch_mux(ek, count, d0, d1, d2, dout, E.stream)
+                        : launch_gates(ek, gate, count, d0, d1, c2s ? d2 : nullptr, nullptr, dout,
+                                       E.stream);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
@@ -520,6 +596,8 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
   std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost(a) > cost(b); });
   std::lock_guard<std::mutex> g(E.mu);
   const size_t per_gate = kScratchWordsPerGate * 8;
+  const bool grow = E.mixed_acc_bytes < total * per_gate || E.mixed_parts_cap < order.size();
+  if (grow && E.mixed_ev_used) HIP_TRY(hipEventSynchronize(E.mixed_ev));
   if (E.mixed_acc_bytes < total * per_gate) {
     if (E.mixed_acc) HIP_TRY(hipFree(E.mixed_acc));
     E.mixed_acc = nullptr;
@@ -546,10 +624,13 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
     args.push_back(a);
   }
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  // the part table and accumulators are free once the previous mixed launch is done
+  if (E.mixed_ev_used) HIP_TRY(hipStreamWaitEvent(s, E.mixed_ev, 0));
   HIP_TRY(hipMemcpyAsync(E.mixed_parts, args.data(), args.size() * sizeof(BootArgs),
                          hipMemcpyHostToDevice, s));
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
-  HIP_TRY(hipEventRecord(E.ev0, s));
+  int rc = timer_start(s);
+  if (rc) return rc;
   bool narrow = true;
   int sets = 0;
   for (const BootArgs& a : args) narrow = narrow && split8_fits(a.br_len, a.br_bits);
@@ -559,12 +640,16 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
   // Measured, 348 BabyBear + 56 Goldilocks + 574 Zama NANDs: 104.5 ms in latency mode
   // against 113.4 ms with one wavefront per gate.
   const bool mixes = (sets & (sets - 1)) != 0;
-  if (first <= E.split_max || (E.split_max > 0 && mixes && first <= E.simd_slots))
+  const uint32_t cutoff = mixed_split_cutoff(E, sets);
+  if (first <= cutoff || (cutoff > 0 && mixes && first <= E.simd_slots))
     HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first,
                                mixed_split_waves(narrow), s));
   else
     HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
-  HIP_TRY(hipEventRecord(E.ev1, s));
+  rc = timer_stop(s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(E.mixed_ev, s));
+  E.mixed_ev_used = true;
   return HB_OK;
 }
 
@@ -577,9 +662,14 @@ int hb_mixed_batch(const hb_batch_part_t* parts, size_t nparts) {
     words += parts[i].count * (uint64_t)(parts[i].ek->p.n + 1) * 4;
   }
   std::vector<hb_batch_part_t> dev(parts, parts + nparts);
+  // one host-buffer mixed call at a time: the staging buffer holds this call's inputs
+  // until its kernel has read them and its outputs until they are copied back
+  static std::mutex host_mu;
+  std::lock_guard<std::mutex> host_lock(host_mu);
   {
     std::lock_guard<std::mutex> g(E.mu);
     if (E.mixed_io_words < words) {
+      if (E.mixed_ev_used) HIP_TRY(hipEventSynchronize(E.mixed_ev));
       if (E.mixed_io) HIP_TRY(hipFree(E.mixed_io));
       E.mixed_io = nullptr;
       E.mixed_io_words = 0;
@@ -653,9 +743,11 @@ int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const
   BootArgs a = make_args(ek, gate, count, d0, c1s ? d1 : nullptr, c2s ? d2 : nullptr,
                          lut ? (const uint64_t*)ek->lut_dev : nullptr, dout, ek->acc_dev);
   a.trace = tr;
-  if (hipEventRecord(E.ev0, E.stream) != hipSuccess || launch_bootstrap(a, p.is_goldilocks, 1, 1, E.stream) != hipSuccess ||
-      hipEventRecord(E.ev1, E.stream) != hipSuccess)
+  if (acc_acquire(ek, E.stream) || timer_start(E.stream))
+    return cleanup(HB_ERR_DEVICE);
+  if (launch_bootstrap(a, p.is_goldilocks, 1, 1, E.stream) != hipSuccess)
     return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
+  if (timer_stop(E.stream) || acc_release(ek, E.stream)) return cleanup(HB_ERR_DEVICE);
   std::vector<uint8_t> host(words * wb);
   if (hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
       hipMemcpyAsync(host.data(), tr, words * wb, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||
@@ -737,11 +829,11 @@ int hb_ntt_dev(int field, int inverse, uint32_t log_n, uint64_t* data, size_t co
   if (rc) return rc;
   Engine& E = engine();
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
-  HIP_TRY(hipEventRecord(E.ev0, s));
+  rc = timer_start(s);
+  if (rc) return rc;
   HIP_TRY(launch_ntt(field, inverse, log_n, data, count, E.tables.tw[field][log_n - 10][inverse ? 1 : 0],
                      E.tables.ninv[field][log_n - 10], s));
-  HIP_TRY(hipEventRecord(E.ev1, s));
-  return HB_OK;
+  return timer_stop(s);
 }
 
 int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t len,
@@ -763,8 +855,15 @@ int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t l
   HIP_TRY(hipMemcpy(host.data(), rgsw_dev, words * 8, hipMemcpyDeviceToHost));
   HostNtt ntt;
   ntt.init(p, 10);
+  // One converted RGSW shared by every call: the previous call's kernel must be done with
+  // it before it is rewritten.
+  static std::mutex conv_mu;
   static void* conv = nullptr;
   static size_t conv_bytes = 0;
+  static hipEvent_t conv_ev = nullptr;
+  std::lock_guard<std::mutex> conv_lock(conv_mu);
+  if (!conv_ev) HIP_TRY(hipEventCreateWithFlags(&conv_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventSynchronize(conv_ev));
   const size_t wb = field ? 8 : 4;
   if (conv_bytes < words * wb) {
     if (conv) (void)hipFree(conv);
@@ -779,10 +878,13 @@ int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t l
     for (size_t k = 0; k < words; k++) w[k] = (uint32_t)to_device_word(p, host[k], ntt.n_inv);
     HIP_TRY(hipMemcpy(conv, w.data(), words * 4, hipMemcpyHostToDevice));
   }
-  HIP_TRY(hipEventRecord(E.ev0, s));
+  rc = timer_start(s);
+  if (rc) return rc;
   HIP_TRY(launch_extprod(field, rlwe_dev, count, conv, len, bits, E.tables.tw[field][0][0],
                          E.tables.tw[field][0][1], s));
-  HIP_TRY(hipEventRecord(E.ev1, s));
+  rc = timer_stop(s);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(conv_ev, s));
   return HB_OK;
 }
 
@@ -811,6 +913,24 @@ int hb_sync(void) {
   HIP_TRY(hipDeviceSynchronize());
   return HB_OK;
 }
+int hb_stream_create(void** out) {
+  if (!out) return fail(HB_ERR_ARG, "null out");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  hipStream_t s = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *out = s;
+  return HB_OK;
+}
+int hb_stream_destroy(void* stream) {
+  if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+  return HB_OK;
+}
+int hb_stream_sync(void* stream) {
+  if (!stream) return fail(HB_ERR_ARG, "null stream");
+  HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  return HB_OK;
+}
 int hb_device_count(int* out) {
   if (!out) return fail(HB_ERR_ARG, "null out");
   int n = 0;
@@ -820,12 +940,11 @@ int hb_device_count(int* out) {
 }
 int hb_last_kernel_ms(double* out) {
   if (!out) return fail(HB_ERR_ARG, "null out");
-  Engine& E = engine();
-  if (!E.ev1) return fail(HB_ERR_STATE, "no kernel launched yet");
-  HIP_TRY(hipEventSynchronize(E.ev1));
+  LaunchTimer& t = launch_timer();
+  if (!t.used) return fail(HB_ERR_STATE, "no kernel launched yet on this thread");
+  HIP_TRY(hipEventSynchronize(t.ev1));
   float ms = 0;
-  HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
-  E.last_ms = ms;
+  HIP_TRY(hipEventElapsedTime(&ms, t.ev0, t.ev1));
   *out = ms;
   return HB_OK;
 }
@@ -897,10 +1016,10 @@ int hb_batch_graph_launch(hb_batch_graph_t* g, void* stream) {
   if (!g) return fail(HB_ERR_ARG, "null graph");
   Engine& E = engine();
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
-  HIP_TRY(hipEventRecord(E.ev0, s));
+  int rc = timer_start(s);
+  if (rc) return rc;
   HIP_TRY(hipGraphLaunch(g->exec, s));
-  HIP_TRY(hipEventRecord(E.ev1, s));
-  return HB_OK;
+  return timer_stop(s);
 }
 
 int hb_batch_graph_destroy(hb_batch_graph_t* g) {
@@ -941,21 +1060,28 @@ int hb_autotune(hb_ek_t* ek, size_t probe_gates, hb_tune_result_t* out) {
   BootArgs a = make_args(ek, GATE_NAND, probe_gates, c0, c1, nullptr, nullptr, o, ek->acc_dev);
   double best = 1e300;
   int best_v = 0;
+  rc = acc_acquire(ek, E.stream);
+  if (rc) return rc;
+  LaunchTimer& t = launch_timer();
   for (int v = 0; v < HB_TUNE_VARIANTS; v++) {
     const int gpw = kVariants[v][0], wps = kVariants[v][1];
     HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, E.stream));  // warm-up
-    HIP_TRY(hipEventRecord(E.ev0, E.stream));
+    rc = timer_start(E.stream);
+    if (rc) return rc;
     HIP_TRY(launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, E.stream));
-    HIP_TRY(hipEventRecord(E.ev1, E.stream));
-    HIP_TRY(hipEventSynchronize(E.ev1));
+    rc = timer_stop(E.stream);
+    if (rc) return rc;
+    HIP_TRY(hipEventSynchronize(t.ev1));
     float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, E.ev0, E.ev1));
+    HIP_TRY(hipEventElapsedTime(&ms, t.ev0, t.ev1));
     out->ms[v] = ms;
     if (ms < best) {
       best = ms;
       best_v = v;
     }
   }
+  rc = acc_release(ek, E.stream);
+  if (rc) return rc;
   E.gpw[ek->p.set] = kVariants[best_v][0];
   E.wps[ek->p.set] = kVariants[best_v][1];
   E.tuned[ek->p.set] = true;
@@ -998,33 +1124,46 @@ int hb_modmul_peak(uint32_t waves, uint32_t iters, double* modmul_per_s, double*
   uint64_t* out = nullptr;
   HIP_TRY(hipMalloc(&out, (size_t)waves * 64 * 8));
   hipError_t e = launch_modmul_peak(out, waves, iters, 7, E.stream);  // warm-up
-  if (e == hipSuccess) e = hipEventRecord(E.ev0, E.stream);
+  if (e == hipSuccess && timer_start(E.stream) != HB_OK) e = hipErrorUnknown;
   if (e == hipSuccess) e = launch_modmul_peak(out, waves, iters, 11, E.stream);
-  if (e == hipSuccess) e = hipEventRecord(E.ev1, E.stream);
-  if (e == hipSuccess) e = hipEventSynchronize(E.ev1);
+  if (e == hipSuccess && timer_stop(E.stream) != HB_OK) e = hipErrorUnknown;
+  LaunchTimer& t = launch_timer();
+  if (e == hipSuccess) e = hipEventSynchronize(t.ev1);
   float ms = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, E.ev0, E.ev1);
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, t.ev0, t.ev1);
   (void)hipFree(out);
   if (e != hipSuccess) return fail(HB_ERR_DEVICE, "modmul peak: %s", hipGetErrorString(e));
-  E.last_ms = ms;
   // per lane and iteration: 4 gl_mac2 = 8 products
   *modmul_per_s = (double)waves * 64.0 * iters * 8.0 / (ms * 1e-3);
   if (ms_out) *ms_out = ms;
   return HB_OK;
 }
 
-int hb_set_latency_mode(uint32_t max_gates) {
+int hb_set_latency_mode_for(int set, uint32_t max_gates) {
+  if (set < 0 || set >= SET_COUNT) return fail(HB_ERR_ARG, "bad parameter set %d", set);
   int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
   if (rc) return rc;
-  engine().split_max = max_gates;
-  engine().split_user = true;
+  Engine& E = engine();
+  E.split_max[set] = max_gates == HB_LATENCY_DEFAULT ? default_split_max(set, E.simd_slots)
+                                                     : max_gates;
   return HB_OK;
 }
 
-int hb_latency_mode(uint32_t* max_gates) {
-  if (!max_gates) return fail(HB_ERR_ARG, "null argument");
-  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
-  if (rc) return rc;
-  *max_gates = engine().split_max;
+int hb_set_latency_mode(uint32_t max_gates) {
+  for (int set = 0; set < SET_COUNT; set++) {
+    int rc = hb_set_latency_mode_for(set, max_gates);
+    if (rc) return rc;
+  }
   return HB_OK;
 }
+
+int hb_latency_mode_for(int set, uint32_t* max_gates) {
+  if (!max_gates) return fail(HB_ERR_ARG, "null argument");
+  if (set < 0 || set >= SET_COUNT) return fail(HB_ERR_ARG, "bad parameter set %d", set);
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  *max_gates = engine().split_max[set];
+  return HB_OK;
+}
+
+int hb_latency_mode(uint32_t* max_gates) { return hb_latency_mode_for(SET_GOLDILOCKS, max_gates); }

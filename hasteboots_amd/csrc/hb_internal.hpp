@@ -36,15 +36,19 @@ struct Engine {
   std::mutex mu;
   int device = -1;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
   DeviceTables tables;
+  // Scratch of mixed-parameter launches (accumulators, the part table the kernel reads
+  // throughout its run, host-call staging). mixed_ev is recorded after the last launch
+  // that used them; the next user's stream waits for it (and a reallocation synchronises
+  // with it), so calls on different streams never share the buffers while a kernel runs.
   void* mixed_acc = nullptr;
   size_t mixed_acc_bytes = 0;
   BootArgs* mixed_parts = nullptr;
   size_t mixed_parts_cap = 0;
   uint64_t* mixed_io = nullptr;
   size_t mixed_io_words = 0;
+  hipEvent_t mixed_ev = nullptr;
+  bool mixed_ev_used = false;
   // Batched-kernel variant per parameter set (HB_SET_*): gates (wavefronts) per
   // workgroup and waves per SIMD. The default is the variant measured fastest on MI355X
   // for Goldilocks; hb_autotune or the cache file replaces it.
@@ -55,17 +59,42 @@ struct Engine {
   // 1-gate-per-workgroup, 1-wave-per-SIMD variant: spreading them over every SIMD beats
   // doubling up waves on half of them.
   uint32_t simd_slots = 1024;
-  // Batches of at most this many untraced gates run in latency mode (split_waves
-  // wavefronts per gate, one workgroup per gate); by default three per CU. Measured on
-  // MI355X (256 CUs), Goldilocks: 256 gates 17.9 ms, 512 35.8 ms, 768 53.8 ms in latency
-  // mode against 67-71 ms with one wavefront per gate, which ties at 1024 gates (72.0
-  // against 71.6 ms); Zama loses there (104 against 113 ms), BabyBear still wins (16.4
-  // against 19.5 ms).
-  uint32_t split_max = 768;
-  // Set by hb_set_latency_mode: split_max then applies to every parameter set (by default
-  // BabyBear's cutoff is one gate per SIMD, see batch_variant).
-  bool split_user = false;
+  // Per parameter set, batches of at most split_max[set] untraced gates run in latency
+  // mode (split_waves wavefronts per gate, one workgroup per gate). Defaults
+  // (default_split_max): three gates per CU for the Goldilocks sets and one per SIMD for
+  // BabyBear. Measured on MI355X (256 CUs), Goldilocks: 256 gates 17.9 ms, 512 35.8 ms,
+  // 768 53.8 ms in latency mode against 67-71 ms with one wavefront per gate, which ties at
+  // 1024 gates (72.0 against 71.6 ms); Zama loses there (104 against 113 ms), BabyBear still
+  // wins (16.4 against 19.5 ms).
+  uint32_t split_max[3] = {1024, 768, 768};
 };
+
+// Built-in latency-mode cutoff of a parameter set on a device with `simd_slots` SIMDs.
+inline uint32_t default_split_max(int set, uint32_t simd_slots) {
+  const uint32_t three_per_cu = 3 * (simd_slots / 4);
+  return set == SET_BABYBEAR ? (simd_slots > three_per_cu ? simd_slots : three_per_cu)
+                             : three_per_cu;
+}
+
+// Latency-mode cutoff of a launch mixing the parameter sets in the bit mask `sets`: the
+// smallest of their cutoffs (0 if any of them has latency mode off).
+inline uint32_t mixed_split_cutoff(const Engine& E, int sets) {
+  uint32_t c = 0xFFFFFFFFu;
+  for (int set = 0; set < SET_COUNT; set++)
+    if (sets & (1 << set)) c = E.split_max[set] < c ? E.split_max[set] : c;
+  return sets ? c : 0;
+}
+
+// Start/stop events of the calling thread's last timed launch (hb_last_kernel_ms reads
+// them); created on first use, one pair per thread.
+struct LaunchTimer {
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool used = false;
+};
+LaunchTimer& launch_timer();
+// Records the start event on `s`; returns HB_OK or an error code.
+int timer_start(hipStream_t s);
+int timer_stop(hipStream_t s);
 
 // Wavefronts per gate in latency mode: 8 (two per SIMD, hiding each other's latency) for
 // the Goldilocks sets, and for any launch of at most one gate per CU; otherwise 4, for
@@ -105,6 +134,14 @@ struct hb_ek {
   size_t io_cap = 0;  // gates
   void* acc_dev = nullptr;
   size_t acc_cap = 0;  // gates
+  // Recorded after every launch that uses acc_dev; a launch on any stream first waits for
+  // it, and ensure_acc synchronises with it before freeing, so kernels of calls on
+  // different streams never share the accumulators.
+  hipEvent_t acc_ev = nullptr;
+  bool acc_ev_used = false;
+  // Mux staging (6 rows of n+1 words per gate), guarded by acc_ev like acc_dev.
+  uint64_t* mux_dev = nullptr;
+  size_t mux_cap = 0;  // gates
 };
 
 namespace hb {

@@ -865,8 +865,34 @@ __global__ __launch_bounds__(64) void modmul_peak_kernel(u64* __restrict__ out, 
 }
 
 // ----------------------------------------------------------------------------
+// Evaluator::not (tfhe/src/bfhe/evaluate.rs:98-104) on `count` LWEs of n1 canonical words:
+// out = -c, then out.b += Q/4, canonical out. One thread per word.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void lwe_not_kernel(const u64* __restrict__ in,
+                                                      u64* __restrict__ out, u32 count, u32 n1,
+                                                      u64 q, int is_goldilocks) {
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= (size_t)count * n1) return;
+  const u64 x = in[k];
+  u64 v = x == 0 ? 0 : q - x;
+  if (k % n1 == n1 - 1) {
+    const u64 q4 = q >> 2;
+    v = is_goldilocks ? Gl::canon(Gl::add(v, q4)) : (v + q4 >= q ? v + q4 - q : v + q4);
+  }
+  out[k] = v;
+}
+
+// ----------------------------------------------------------------------------
 // Launchers
 // ----------------------------------------------------------------------------
+hipError_t launch_lwe_not(const u64* in, u64* out, u32 count, u32 n1, u64 q, int is_goldilocks,
+                          hipStream_t s) {
+  const size_t words = (size_t)count * n1;
+  if (words == 0) return hipSuccess;
+  hipLaunchKernelGGL(lwe_not_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, in,
+                     out, count, n1, q, is_goldilocks);
+  return hipGetLastError();
+}
 template <class F, bool D8>
 static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStream_t s) {
   const dim3 g1(a.count), g2((a.count + 1) / 2), g4((a.count + 3) / 4);

@@ -15,7 +15,8 @@ enum GateKind : int {
   GATE_XOR = 4,
   GATE_XNOR = 5,
   GATE_MAJORITY = 6,
-  GATE_BOOTSTRAP = 7  // c0 is the ciphertext, `lut` the test polynomial
+  GATE_BOOTSTRAP = 7,  // c0 is the ciphertext, `lut` the test polynomial
+  GATE_MUX = 8         // host-side sequence: NOT, two ANDs, OR-LUT bootstrap of the sum
 };
 
 struct BootArgs {
@@ -59,6 +60,9 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t to
                               int waves, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
+// Evaluator::not on `count` LWEs of n1 canonical words (out may equal in).
+hipError_t launch_lwe_not(const uint64_t* in, uint64_t* out, uint32_t count, uint32_t n1,
+                          uint64_t q, int is_goldilocks, hipStream_t s);
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
                       const void* tw, uint64_t ninv, hipStream_t s);
 // Goldilocks MAC peak microbench: `waves` wavefronts, `iters` x 4 gl_mac2 per lane;

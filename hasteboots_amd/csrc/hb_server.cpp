@@ -254,12 +254,14 @@ int launch(hb_server* S, Slot& sl) {
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
   bool any_traced = false, narrow = true;
+  int sets = 0;
   for (uint32_t k = 0; k < nparts; k++) {
     const BootArgs& a = sl.parts_host[k];
     any_traced |= a.trace != nullptr;
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
-  if (!any_traced && used <= engine().split_max)
+  for (const auto& w : sl.work) sets |= 1 << w.first->ek->p.set;
+  if (!any_traced && used <= mixed_split_cutoff(engine(), sets))
     HIP_TRY(launch_mixed_split(sl.parts_dev, (int)nparts, used, mixed_split_waves(narrow),
                                sl.s));
   else

@@ -19,10 +19,11 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import TUNE_VARIANTS, HbParams, HbTuneResult, check, lib, ptr
+from ._lib import LATENCY_DEFAULT, TUNE_VARIANTS, HbParams, HbTuneResult, check, lib, ptr
 
 SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA = 0, 1, 2
-GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6}
+GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6,
+         "mux": 8}
 
 
 @dataclass(frozen=True)
@@ -235,14 +236,9 @@ class Evaluator:
         return LWECiphertext(out)
 
     def mux(self, c0, c1, c2):
-        """evaluate.rs:240-263: (c0 AND c1) + (NOT c0 AND c2), then OR-bootstrap."""
-        nc0 = self.not_(c0)
-        t = self.gate_batch("and", np.stack([c0.words, nc0.words]),
-                            np.stack([c1.words, c2.words]))
-        q = self.params.q
-        s = ((t[0].astype(object) + t[1].astype(object)) % q).astype(np.uint64)
-        lut = self.lut("or")
-        return self.bootstrap(LWECiphertext(s), lut)
+        """evaluate.rs:245-263: if c0 then c1 else c2, as (c0 AND c1) + (NOT c0 AND c2)
+        bootstrapped with the OR LUT (hb_gate with HB_GATE_MUX, all on the device)."""
+        return self._gate("mux", c0, c1, c2)
 
     def lut(self, kind: str) -> np.ndarray:
         """Test polynomials of evaluate.rs:267-465 (lut_step = 1), canonical words."""
@@ -359,7 +355,9 @@ class BatchGraph:
         self._g = g
 
     def set_inputs(self, c0s, c1s, c2s=None):
+        """Stages host ciphertexts; waits for replays in flight (which read the buffers)."""
         L = lib()
+        check(L.hb_sync())
         for buf, a in ((self._bufs[0], c0s), (self._bufs[1], c1s),
                        (self._bufs[3] if c2s is not None else None, c2s)):
             if a is None:
@@ -372,7 +370,10 @@ class BatchGraph:
         check(lib().hb_batch_graph_launch(self._g, stream))
 
     def outputs(self) -> np.ndarray:
+        """Results of the last replay (waits for it: replays run on a non-blocking stream,
+        which the copy's null stream does not wait for by itself)."""
         out = np.zeros((self.count, self.n1), dtype=np.uint64)
+        check(lib().hb_sync())
         check(lib().hb_memcpy_d2h(out.ctypes.data, self._bufs[2], out.nbytes))
         return out
 
@@ -407,15 +408,30 @@ def set_tuned_config(set_id: int, gates_per_workgroup: int, waves_per_simd: int)
     check(lib().hb_set_tuned_config(set_id, gates_per_workgroup, waves_per_simd))
 
 
-def latency_mode() -> int:
-    """Largest batch that runs 4 wavefronts per gate (0: never)."""
+def latency_mode(set_id: int = SET_GOLDILOCKS) -> int:
+    """Largest batch of the parameter set that runs in latency mode (0: never)."""
     v = C.c_uint32()
-    check(lib().hb_latency_mode(C.byref(v)))
+    check(lib().hb_latency_mode_for(set_id, C.byref(v)))
     return v.value
 
 
-def set_latency_mode(max_gates: int) -> None:
-    check(lib().hb_set_latency_mode(max_gates))
+def set_latency_mode(max_gates: int, set_id: int | None = None) -> None:
+    """Latency-mode cutoff of one parameter set, or of all when set_id is None;
+    max_gates = LATENCY_DEFAULT restores the built-in cutoff."""
+    if set_id is None:
+        check(lib().hb_set_latency_mode(max_gates))
+    else:
+        check(lib().hb_set_latency_mode_for(set_id, max_gates))
+
+
+def latency_modes() -> tuple[int, int, int]:
+    """The cutoffs of all three parameter sets (for restore_latency_modes)."""
+    return tuple(latency_mode(s) for s in (SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA))
+
+
+def restore_latency_modes(modes) -> None:
+    for s, m in zip((SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA), modes):
+        set_latency_mode(m, s)
 
 
 def mixed_batch(parts) -> list[np.ndarray]:

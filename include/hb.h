@@ -8,8 +8,12 @@
  *     returns Err where these return a negative code.
  *   - Words are little-endian uint64_t canonical field values (in [0, Q)), for all
  *     parameter sets. An LWE ciphertext is n+1 words: a[0..n) then b.
- *   - Handles are opaque and library-owned. hb_ek_t is read-only after creation
- *     and may be shared across threads; hb_sk_t carries an encryption PRG and is
+ *   - Handles are opaque and library-owned. hb_ek_t may be shared across threads
+ *     and streams: its keys are read-only after creation, and the per-key scratch of
+ *     the batched kernels is handed from one launch to the next by a device event, so
+ *     calls on different streams are ordered on the device where they share it (they
+ *     do not overlap) and never corrupt each other; the same holds for the engine-wide
+ *     scratch of mixed-parameter launches. hb_sk_t carries an encryption PRG and is
  *     not thread-safe (the reference keeps its PRG in a RefCell,
  *     fhe_core/src/secret_key.rs:55).
  *   - Entry points with a `_dev` suffix take device pointers on the engine's GPU
@@ -42,8 +46,11 @@ typedef enum {
   HB_SET_ZAMA = 2        /* ZAMA_BINARY_128_BITS_PARAMETERS (= vfhe ZAMA_GOLDILOCKS_PARAMETERS) */
 } hb_set_id;
 
-/* Gate kinds: tfhe::Evaluator::{nand, and, or, nor, xor, xnor, majority}
- * (tfhe/src/bfhe/evaluate.rs:113-231). */
+/* Gate kinds: tfhe::Evaluator::{nand, and, or, nor, xor, xnor, majority, mux}
+ * (tfhe/src/bfhe/evaluate.rs:113-263). Value 7 is reserved (hb_trace_expand uses it for
+ * hb_bootstrap_traced gates). Mux takes three inputs (select, then, else) and costs three
+ * bootstraps: two AND gates in one launch, their sum, then an OR-LUT bootstrap
+ * (evaluate.rs:245-263). */
 typedef enum {
   HB_GATE_NAND = 0,
   HB_GATE_AND = 1,
@@ -51,7 +58,8 @@ typedef enum {
   HB_GATE_NOR = 3,
   HB_GATE_XOR = 4,
   HB_GATE_XNOR = 5,
-  HB_GATE_MAJORITY = 6
+  HB_GATE_MAJORITY = 6,
+  HB_GATE_MUX = 8
 } hb_gate_kind;
 
 typedef struct {
@@ -107,7 +115,8 @@ int hb_ek_params(const hb_ek_t* ek, hb_params_t* out);
 
 /* Evaluator::nand (tfhe/src/bfhe/evaluate.rs:113-121): one gate, one launch. */
 int hb_nand(hb_ek_t* ek, const uint64_t* c0, const uint64_t* c1, uint64_t* out);
-/* Evaluator::{and, or, nor, xor, xnor, majority} (evaluate.rs:129-231); c2 only for majority. */
+/* Evaluator::{and, or, nor, xor, xnor, majority, mux} (evaluate.rs:129-263); c2 only for
+ * majority and mux. */
 int hb_gate(hb_ek_t* ek, int gate, const uint64_t* c0, const uint64_t* c1, const uint64_t* c2,
             uint64_t* out);
 /* Evaluator::bootstrap (evaluate.rs:86-88 → :42-60) with an arbitrary LUT of N words. */
@@ -117,7 +126,8 @@ int hb_not(const hb_ek_t* ek, const uint64_t* c, uint64_t* out);
 
 /* Batched gates: `count` independent gates of one kind in a single launch (no
  * reference counterpart; the reference runs gates one per thread, tfhe/examples/bfhe.rs:139-148).
- * Host arrays of count*(n+1) words; c2s only for majority. */
+ * Host arrays of count*(n+1) words; c2s only for majority and mux (mux: two launches,
+ * 2*count AND gates then count OR-LUT bootstraps). */
 int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,
                   const uint64_t* c2s, uint64_t* outs);
 /* Same on device-resident arrays; `stream` is a hipStream_t or NULL. Asynchronous. */
@@ -163,12 +173,17 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * graphs) of at most max_gates gates run 8 wavefronts per gate (4 for BabyBear launches of
  * more than one gate per compute unit), each wave taking a share of the digit transforms
  * and of the inverse transforms, which cuts a gate's latency about 4x at lower peak
- * throughput. Default: three gates per compute unit (768 on MI355X, where latency mode
- * stays faster than one wavefront per gate), and one per SIMD (1024) for BabyBear, whose
- * latency mode stays ahead that far. A call sets one cutoff for every parameter set;
- * 0 disables. hb_latency_mode reports the cutoff of the Goldilocks sets. */
+ * throughput. Each parameter set has its own cutoff. Defaults: three gates per compute unit
+ * (768 on MI355X, where latency mode stays faster than one wavefront per gate) for the
+ * Goldilocks sets, and one per SIMD (1024) for BabyBear, whose latency mode stays ahead
+ * that far. hb_set_latency_mode sets the cutoff of every set, hb_set_latency_mode_for of
+ * one; 0 disables latency mode, HB_LATENCY_DEFAULT restores the built-in cutoff.
+ * hb_latency_mode reports the Goldilocks cutoff, hb_latency_mode_for any set's. */
+#define HB_LATENCY_DEFAULT 0xFFFFFFFFu
 int hb_set_latency_mode(uint32_t max_gates);
+int hb_set_latency_mode_for(int set, uint32_t max_gates);
 int hb_latency_mode(uint32_t* max_gates);
+int hb_latency_mode_for(int set, uint32_t* max_gates);
 
 /* Mixed-parameter batch: several parts, each with its own evaluation key (any of the
  * three parameter sets) and gate kind, bootstrapped in ONE kernel launch. No reference
@@ -296,9 +311,15 @@ int hb_memcpy_h2d(void* dst_dev, const void* src, size_t bytes);
 int hb_memcpy_d2h(void* dst, const void* src_dev, size_t bytes);
 int hb_sync(void);
 int hb_device_count(int* out);
+/* Streams on the engine's device (default priority, non-blocking), for callers without a
+ * HIP runtime of their own. */
+int hb_stream_create(void** out);
+int hb_stream_destroy(void* stream);
+int hb_stream_sync(void* stream);
 
-/* Average duration in ms of the last `hb_*_dev` kernel launch, measured with HIP events
- * recorded on the stream the kernel ran on. */
+/* Duration in ms of the calling thread's last timed launch (hb_*_dev, host-buffer gate
+ * calls, hb_batch_graph_launch, hb_ntt_dev, hb_external_product_dev), measured with HIP
+ * events recorded on the stream the kernel ran on; each thread has its own events. */
 int hb_last_kernel_ms(double* out);
 
 /* Human-readable message for the last error on the calling thread. */

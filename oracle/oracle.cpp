@@ -621,7 +621,8 @@ static u32 log2u(u32 x) {
 }
 
 // Gate pre-processing (tfhe/src/bfhe/evaluate.rs:98-263) on raw words.
-enum Gate { G_NAND = 0, G_AND = 1, G_OR = 2, G_NOR = 3, G_XOR = 4, G_XNOR = 5, G_MAJ = 6 };
+enum Gate { G_NAND = 0, G_AND = 1, G_OR = 2, G_NOR = 3, G_XOR = 4, G_XNOR = 5, G_MAJ = 6,
+            G_MUX = 8 };
 template <class F>
 static int gate_combine(const Params& p, int gate, const typename F::word* c0,
                         const typename F::word* c1, const typename F::word* c2,
@@ -648,6 +649,15 @@ static int gate_combine(const Params& p, int gate, const typename F::word* c0,
   return -1;
 }
 
+// Evaluator::not (tfhe/src/bfhe/evaluate.rs:98-104): LWE::neg (component-wise field neg),
+// then b += Q/4 (field add).
+template <class F>
+static void lwe_not(const Params& p, const typename F::word* c, typename F::word* out) {
+  const u32 n1 = p.n + 1;
+  for (u32 k = 0; k < n1; k++) out[k] = F::neg(c[k]);
+  out[p.n] = F::add(out[p.n], F::from_value(p.q >> 2));
+}
+
 template <class F>
 static void run_gates(const Ek<F>& e, int gate, size_t count, const u64* c0s, const u64* c1s,
                       const u64* c2s, u64* outs, int threads) {
@@ -667,8 +677,22 @@ static void run_gates(const Ek<F>& e, int gate, size_t count, const u64* c0s, co
         if (c2s) c2[k] = F::from_value(c2s[g * n1 + k]);
       }
       int kind = 0;
-      gate_combine<F>(p, gate, a.data(), b.data(), c2s ? c2.data() : nullptr, c.data(), &kind);
-      build_lut<F>(p, kind, lut.data());
+      if (gate == G_MUX) {
+        // Evaluator::mux (evaluate.rs:245-263): t0 = and(c0, c1), t1 = and(not(c0), c2),
+        // t0 += t1 (add_inplace_component_wise), then the OR-LUT bootstrap of t0.
+        std::vector<typename F::word> nc0(n1), t0(n1), t1(n1);
+        lwe_not<F>(p, a.data(), nc0.data());
+        gate_combine<F>(p, G_AND, a.data(), b.data(), nullptr, c.data(), &kind);
+        build_lut<F>(p, kind, lut.data());
+        bootstrap<F>(p, e.bsk.data(), e.ksk.data(), c.data(), lut.data(), t0.data(), s, T);
+        gate_combine<F>(p, G_AND, nc0.data(), c2.data(), nullptr, c.data(), &kind);
+        bootstrap<F>(p, e.bsk.data(), e.ksk.data(), c.data(), lut.data(), t1.data(), s, T);
+        for (u32 k = 0; k < n1; k++) c[k] = F::add(t0[k], t1[k]);
+        build_lut<F>(p, LUT_OR, lut.data());
+      } else {
+        gate_combine<F>(p, gate, a.data(), b.data(), c2s ? c2.data() : nullptr, c.data(), &kind);
+        build_lut<F>(p, kind, lut.data());
+      }
       bootstrap<F>(p, e.bsk.data(), e.ksk.data(), c.data(), lut.data(), out.data(), s, T);
       for (u32 k = 0; k < n1; k++) outs[g * n1 + k] = F::value(out[k]);
     }

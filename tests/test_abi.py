@@ -56,6 +56,11 @@ def test_tuning_and_launch_arguments_are_checked_before_the_device(hb):
     g, w = C.c_int(), C.c_int()
     assert l.hb_tuned_config(5, C.byref(g), C.byref(w), None) == -1
     assert l.hb_latency_mode(None) == -1
+    assert l.hb_latency_mode_for(1, None) == -1
+    assert l.hb_set_latency_mode_for(3, 10) == -1
+    assert b"parameter set" in l.hb_last_error()
+    assert l.hb_gate_batch_dev(None, 8, 4, None, None, None, None, None) == -1
+    assert l.hb_stream_sync(None) == -1
     h = C.c_void_p()
     assert l.hb_server_create(0, 0, 0, C.byref(h)) == -1
     assert b"max_gates" in l.hb_last_error()

@@ -126,14 +126,14 @@ def test_latency_mode_wave_counts(hb, orc, keyset, set_id, count):
         _EVALUATORS[set_id] = hb.Evaluator(keys=(bsk, ksk), params=P)
     ev = _EVALUATORS[set_id]
     b0, b1, c0, c1 = _inputs_for(hb, sk, set_id, 91, count)
-    old = hb.latency_mode()
+    old = hb.latency_modes()
     try:
         hb.set_latency_mode(1 << 20)
         split = ev.gate_batch("nand", c0, c1)
         hb.set_latency_mode(0)
         single = ev.gate_batch("nand", c0, c1)
     finally:
-        hb.set_latency_mode(old)
+        hb.restore_latency_modes(old)
     assert np.array_equal(split, single)
     sample = np.r_[0:3, count - 3:count]
     ref = orc.OracleKeys(set_id, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=6)
@@ -183,14 +183,14 @@ def test_latency_mode_matches_wave_per_gate(hb, orc, gold):
     P, sk, bsk, ksk, ev = gold
     count = 70
     b0, b1, c0, c1 = _inputs(hb, sk, 77, count)
-    old = hb.latency_mode()
+    old = hb.latency_modes()
     try:
         hb.set_latency_mode(1 << 20)
         split = ev.gate_batch("nand", c0, c1)
         hb.set_latency_mode(0)
         single = ev.gate_batch("nand", c0, c1)
     finally:
-        hb.set_latency_mode(old)
+        hb.restore_latency_modes(old)
     assert np.array_equal(split, single)
     ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[:8], c1[:8], threads=8)
     assert np.array_equal(split[:8], ref)

@@ -70,14 +70,14 @@ def test_mixed_latency_mode_and_wave_per_gate_agree(hb, orc, evaluators):
         c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
         parts.append((ev, gate, c0, c1, None))
         meta.append((s, gate, c0, c1))
-    old = hb.latency_mode()
+    old = hb.latency_modes()
     try:
         hb.set_latency_mode(1 << 20)
         split = hb.mixed_batch(parts)
         hb.set_latency_mode(0)
         single = hb.mixed_batch(parts)
     finally:
-        hb.set_latency_mode(old)
+        hb.restore_latency_modes(old)
     for (s, gate, c0, c1), a, b in zip(meta, split, single):
         assert np.array_equal(a, b), (s, gate)
         P, sk, bsk, ksk, ev = evaluators[s]

@@ -186,12 +186,12 @@ def test_goldilocks_raw_word_accumulator(hb, orc, latency):
     land in [P, 2^64). GPU and oracle must agree word for word, both through the
     latency-mode kernel and through the one-wavefront-per-gate kernel with byte digits."""
     say(f"raw-word accumulator, latency mode {latency}")
-    old = hb.latency_mode()
-    hb.set_latency_mode(old if latency else 0)
+    old = hb.latency_modes()
+    hb.restore_latency_modes(old if latency else (0, 0, 0))
     try:
         _raw_word_accumulator(hb, orc)
     finally:
-        hb.set_latency_mode(old)
+        hb.restore_latency_modes(old)
 
 
 def _raw_word_accumulator(hb, orc):

@@ -95,15 +95,17 @@ def test_oversized_circuit_is_rejected(hb, clients):
     srv.close()
 
 
-def test_traced_request_traces_match_oracle(hb, orc, keyset):
+@pytest.mark.parametrize("set_id", [0, 1], ids=["babybear", "goldilocks"])
+def test_traced_request_traces_match_oracle(hb, orc, keyset, set_id):
     """A traced circuit served next to an untraced one: every gate's emitted core trace,
     expanded, equals the oracle's traced records for that gate (rounds 0, 363, 727 and the
     tail), and the outputs equal the oracle's traced evaluation."""
     from hasteboots_amd.serving import Server, random_nand_circuit
-    P, sk, bsk, ksk = keyset(hb.SET_BABYBEAR)
+    say(f"traced serving, set {set_id}")
+    P, sk, bsk, ksk = keyset(set_id)
     ev = hb.Evaluator(keys=(bsk, ksk), params=P)
     enc = hb.Encryptor(sk)
-    rng = stream(54, "traced")
+    rng = stream(54 + set_id, "traced")
     circ = random_nand_circuit(rng, 2, 3)
     bits = rng.integers(0, 2, size=3)
     cts = enc.encrypt_many(bits)
@@ -114,8 +116,8 @@ def test_traced_request_traces_match_oracle(hb, orc, keyset):
     out, _, core = srv.wait(r)
     srv.wait(r_other)
     srv.close()
-    ok = orc.OracleKeys(hb.SET_BABYBEAR, bsk, ksk)
-    lut = orc.build_lut(orc.params(hb.SET_BABYBEAR), 0)
+    ok = orc.OracleKeys(set_id, bsk, ksk)
+    lut = orc.build_lut(orc.params(set_id), 0)
     rounds = [0, 363, 727]
     wires = list(cts)
     for g in range(circ.n_gates):
@@ -149,3 +151,43 @@ def test_server_matches_oracle_gate_by_gate(hb, orc, keyset):
         wires.append(r)
     ref = np.stack([wires[o] for o in circ.outputs])
     assert np.array_equal(out, ref)
+
+
+def test_config_size_circuits_alone_equal_mixed(hb, orc, clients):
+    """Circuits drawn as the serving workload draws them (depth uniform in 1-64, width in
+    2-64, NAND only, Goldilocks keys of three clients): each circuit's outputs are the same
+    words alone and submitted together, decrypt to the plaintext evaluation, and one
+    moderate circuit matches the oracle gate by gate."""
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    rng = stream(55, "config-size")
+    jobs = []
+    for i in range(3):
+        ev, enc, dec = clients[i]
+        circ = random_nand_circuit(rng, int(rng.integers(1, 65)), int(rng.integers(2, 65)))
+        bits = rng.integers(0, 2, size=circ.n_inputs)
+        jobs.append((ev, dec, circ, bits, enc.encrypt_many(bits)))
+    say("config-size circuits: " + ", ".join(f"{j[2].depth()}x{j[2].n_inputs}" for j in jobs))
+    srv = Server(max_gates_per_launch=256)
+    alone = [srv.wait(srv.submit(ev, circ, cts))[0] for ev, dec, circ, bits, cts in jobs]
+    ids = [srv.submit(ev, circ, cts) for ev, dec, circ, bits, cts in jobs]
+    mixed = [srv.wait(r)[0] for r in ids]
+    srv.close()
+    for (ev, dec, circ, bits, cts), a, m in zip(jobs, alone, mixed):
+        assert np.array_equal(a, m)
+        got = [dec.decrypt(hb.LWECiphertext(m[k])) for k in range(m.shape[0])]
+        assert got == list(circ.eval_plain(bits))
+    # one 6-layer, 16-wide circuit gate by gate against the oracle (a layer per oracle call)
+    ev, enc, dec = clients[1]
+    circ = random_nand_circuit(rng, 6, 16)
+    bits = rng.integers(0, 2, size=16)
+    cts = enc.encrypt_many(bits)
+    srv = Server(max_gates_per_launch=256)
+    out, _ = srv.wait(srv.submit(ev, circ, cts))
+    srv.close()
+    bsk, ksk = ev.export_keys()
+    ok = orc.OracleKeys(hb.SET_GOLDILOCKS, bsk, ksk)
+    wires = np.concatenate([cts, np.zeros((circ.n_gates, cts.shape[1]), np.uint64)])
+    for layer in range(6):
+        g = np.arange(16 * layer, 16 * (layer + 1))
+        wires[16 + g] = ok.gate_batch(0, wires[circ.in0[g]], wires[circ.in1[g]], threads=16)
+    assert np.array_equal(out, wires[circ.outputs])
