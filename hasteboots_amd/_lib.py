@@ -85,6 +85,8 @@ _PROTOS = {
     "hb_gate_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, u64p, u64p, u64p, u64p]),
     "hb_gate_batch_dev": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hb_gate_batch_dev_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "hb_batch_graph_create": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     "hb_batch_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),

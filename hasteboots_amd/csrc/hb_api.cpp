@@ -22,6 +22,12 @@ using namespace hb;
 
 namespace {
 thread_local std::string g_err;
+
+// Per-gate scratch of hb_gate_batch_dev_noalloc (accumulator and byte digits, 3N words of
+// the set's word size per gate), reserved on every GPU the library runs on for the life of
+// the process. Host code uses the address hipGetSymbolAddress returns (Engine::
+// noalloc_scratch), never this name.
+static __device__ __align__(256) unsigned char scratch[HB_NOALLOC_SCRATCH_BYTES];
 }  // namespace
 
 namespace hb {
@@ -177,6 +183,13 @@ int hb::ensure_engine(int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
     for (int set = 0; set < SET_COUNT; set++) E.split_max[set] = default_split_max(set, E.simd_slots);
+  }
+  {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(HB_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    E.noalloc_scratch = p;
   }
   load_tune_cache(E);
   return HB_OK;
@@ -541,6 +554,36 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0, c
   std::lock_guard<std::mutex> g(ek->mu);
   if (gate == GATE_MUX) return launch_mux(ek, count, c0, c1, c2, outs, s);
   return launch_gates(ek, gate, count, c0, c1, c2, nullptr, outs, s);
+}
+
+int hb_gate_batch_dev_noalloc(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0,
+                              const uint64_t* c1, const uint64_t* c2, uint64_t* outs,
+                              void* stream) {
+  if (!ek || !c0 || !c1 || !outs) return fail(HB_ERR_ARG, "null argument");
+  if (gate < 0 || gate > GATE_MAJORITY) return fail(HB_ERR_ARG, "bad gate %d", gate);
+  if (gate == GATE_MAJORITY && !c2) return fail(HB_ERR_ARG, "majority needs three inputs");
+  if (count == 0) return HB_OK;
+  const ParamSet& p = ek->p;
+  const size_t per_gate = kScratchWordsPerGate * (p.is_goldilocks ? 8 : 4);
+  const size_t max_gates = HB_NOALLOC_SCRATCH_BYTES / per_gate;
+  if (count > max_gates)
+    return fail(HB_ERR_CAPACITY, "%zu gates exceed the fixed scratch (%zu gates of this set)",
+                count, max_gates);
+  Engine& E = engine();
+  if (!E.noalloc_scratch) return fail(HB_ERR_STATE, "engine not set up");
+  hipStream_t s = stream ? (hipStream_t)stream : E.stream;
+  // The kernels write every scratch word they read (the accumulator is initialised from
+  // the LUT, the digits from the rotated accumulator), so nothing is cleared first.
+  BootArgs a = make_args(ek, gate, count, c0, c1, c2, nullptr, outs, E.noalloc_scratch);
+  int gpw, wps;
+  batch_variant(E, p.set, count, &gpw, &wps);
+  if (gpw == 0)
+    HIP_TRY(launch_bootstrap_split(
+        a, p.is_goldilocks,
+        split_waves(E, p.is_goldilocks, count, split8_fits(p.br_len, p.br_bits)), s));
+  else
+    HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
+  return HB_OK;
 }
 
 int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const uint64_t* c1s,

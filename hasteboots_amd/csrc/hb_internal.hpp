@@ -37,6 +37,9 @@ struct Engine {
   int device = -1;
   hipStream_t stream = nullptr;
   DeviceTables tables;
+  // Device address of the fixed scratch array of hb_gate_batch_dev_noalloc on this
+  // engine's device (hipGetSymbolAddress at engine set-up).
+  void* noalloc_scratch = nullptr;
   // Scratch of mixed-parameter launches (accumulators, the part table the kernel reads
   // throughout its run, host-call staging). mixed_ev is recorded after the last launch
   // that used them; the next user's stream waits for it (and a reallocation synchronises

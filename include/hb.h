@@ -135,6 +135,21 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_d
                       const uint64_t* c1s_dev, const uint64_t* c2s_dev, uint64_t* outs_dev,
                       void* stream);
 
+/* No reference counterpart. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): hb_gate_batch_dev
+ * for gate kinds 0-6 with the kernels' per-gate scratch (accumulator and byte digits, 3N
+ * words of the set's word size per gate) taken from a fixed device array of
+ * HB_NOALLOC_SCRATCH_BYTES that the library reserves on its GPU for the life of the
+ * process; the call allocates nothing. At most 1024 Goldilocks / Zama or 2048 BabyBear gates
+ * per call (HB_ERR_CAPACITY above). Every call on the GPU shares that one array, which keeps
+ * the last call's contents: calls whose kernels can overlap (other streams, other threads
+ * without ordering) race and corrupt each other's results, so the caller serialises them.
+ * Not timed by hb_last_kernel_ms. */
+#define HB_NOALLOC_SCRATCH_BYTES (24u << 20)
+int hb_gate_batch_dev_noalloc(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_dev,
+                              const uint64_t* c1s_dev, const uint64_t* c2s_dev,
+                              uint64_t* outs_dev, void* stream);
+
 /* Captured batch launch: hb_gate_batch_dev for fixed device buffers recorded once as a
  * hipGraph (with its own accumulator scratch) and replayed by hb_batch_graph_launch
  * without per-call host work. Inputs may be rewritten between launches; replays on one
