@@ -875,7 +875,7 @@ int hb_ntt_dev(int field, int inverse, uint32_t log_n, uint64_t* data, size_t co
   rc = timer_start(s);
   if (rc) return rc;
   HIP_TRY(launch_ntt(field, inverse, log_n, data, count, E.tables.tw[field][log_n - 10][inverse ? 1 : 0],
-                     E.tables.ninv[field][log_n - 10], s));
+                     E.tables.ninv[field][log_n - 10], E.simd_slots / 4, s));
   return timer_stop(s);
 }
 

@@ -753,51 +753,81 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
 }
 
 // ----------------------------------------------------------------------------
-// Microbench: negacyclic NTT over `count` polynomials, one wave per polynomial,
-// 4 waves per workgroup. Forward: natural order in, slot order out. Inverse: slot
-// order in, natural order out, times 1/N. Canonical u64 words in and out.
+// Microbench: negacyclic NTT over `count` polynomials. Forward: natural order in, slot
+// order out. Inverse: slot order in, natural order out, times 1/N. Canonical u64 words in
+// and out. Persistent waves, WAVES per workgroup sharing one LDS twiddle table: wave w of
+// the grid transforms polynomials w, w + W, w + 2W, ... (W = waves in the grid), and loads
+// the next one into registers before transforming the current one, so the HBM reads of
+// one polynomial overlap the arithmetic of the previous (the Goldilocks kernels hold 256
+// VGPRs, two waves per SIMD, too few for wave switching alone to hide the loads).
+// HALF (N = 2048): exchanges through half-size buffers (ntt_fwd_wave). PIPE: the
+// next-polynomial prefetch (off for N = 2048, whose 64 extra VGPRs would halve the waves).
 // ----------------------------------------------------------------------------
-template <class F, int E>
-__global__ __launch_bounds__(256) void ntt_kernel(u64* __restrict__ data, size_t count, int inverse,
-                                                  const void* tw_, typename DF<F>::W ninv) {
+template <class F, int E, bool INV>
+__device__ __forceinline__ void ntt_load(const u64* __restrict__ d, int lane, typename DF<F>::W (&x)[E]) {
   using W = typename DF<F>::W;
-  constexpr int N = 64 * E;
-  __shared__ W lds_all[4][N + N / 16];
-  __shared__ W twl[N];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const W* tw = static_cast<const W*>(tw_);
-  for (int k = threadIdx.x; k < N; k += 256) twl[k] = tw[k];
-  __syncthreads();
-  const size_t poly = (size_t)blockIdx.x * 4 + wave;
-  if (poly >= count) return;
-  W* lds = lds_all[wave];
-  u64* d = data + poly * N;
-  W x[E];
-  if (!inverse) {
+  if constexpr (!INV) {
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (W)d[lane + 64 * e];
-    ntt_fwd_wave<F, E>(x, lds, twl, lane);
-#pragma unroll
-    for (int q = 0; q < E / 4; q++) {
-      const int p = 4 * (lane + 64 * q);
-      ulonglong2* o = reinterpret_cast<ulonglong2*>(d + p);
-      o[0] = make_ulonglong2((u64)DF<F>::canon(x[4 * q]), (u64)DF<F>::canon(x[4 * q + 1]));
-      o[1] = make_ulonglong2((u64)DF<F>::canon(x[4 * q + 2]), (u64)DF<F>::canon(x[4 * q + 3]));
-    }
   } else {
 #pragma unroll
     for (int q = 0; q < E / 4; q++) {
-      const int p = 4 * (lane + 64 * q);
-      const ulonglong2* s = reinterpret_cast<const ulonglong2*>(d + p);
-      ulonglong2 a = s[0], b = s[1];
+      const ulonglong2* s = reinterpret_cast<const ulonglong2*>(d + 4 * (lane + 64 * q));
+      const ulonglong2 a = s[0], b = s[1];
       x[4 * q] = (W)a.x;
       x[4 * q + 1] = (W)a.y;
       x[4 * q + 2] = (W)b.x;
       x[4 * q + 3] = (W)b.y;
     }
-    ntt_inv_wave<F, E>(x, lds, twl, lane);
+  }
+}
+
+template <class F, int E, int WAVES, bool HALF, bool PIPE, bool INV>
+__global__ __launch_bounds__(64 * WAVES) void ntt_kernel(u64* __restrict__ data, size_t count,
+                                                         const void* tw_, typename DF<F>::W ninv) {
+  using W = typename DF<F>::W;
+  constexpr int N = 64 * E;
+  constexpr int BUF = HALF ? 1024 + 64 : N + N / 16;
+  __shared__ W lds_all[WAVES][BUF];
+  __shared__ W twl[N];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const W* tw = static_cast<const W*>(tw_);
+  for (int k = threadIdx.x; k < N; k += 64 * WAVES) twl[k] = tw[k];
+  __syncthreads();
+  const size_t stride = (size_t)gridDim.x * WAVES;
+  size_t poly = (size_t)blockIdx.x * WAVES + wave;
+  if (poly >= count) return;
+  W* lds = lds_all[wave];
+  W x[E], nx[PIPE ? E : 1];
+  ntt_load<F, E, INV>(data + poly * N, lane, x);
+  for (; poly < count; poly += stride) {
+    const bool more = poly + stride < count;  // wave-uniform
+    if constexpr (PIPE) {
+      if (more) ntt_load<F, E, INV>(data + (poly + stride) * N, lane, nx);
+      sched_fence();
+    }
+    u64* d = data + poly * N;
+    if constexpr (!INV) {
+      ntt_fwd_wave<F, E, false, true, HALF>(x, lds, twl, lane);
 #pragma unroll
-    for (int e = 0; e < E; e++) d[lane + 64 * e] = (u64)DF<F>::canon(DF<F>::mul(x[e], ninv));
+      for (int q = 0; q < E / 4; q++) {
+        ulonglong2* o = reinterpret_cast<ulonglong2*>(d + 4 * (lane + 64 * q));
+        o[0] = make_ulonglong2((u64)DF<F>::canon(x[4 * q]), (u64)DF<F>::canon(x[4 * q + 1]));
+        o[1] = make_ulonglong2((u64)DF<F>::canon(x[4 * q + 2]), (u64)DF<F>::canon(x[4 * q + 3]));
+      }
+    } else {
+      ntt_inv_wave<F, E, true, HALF>(x, lds, twl, lane);
+#pragma unroll
+      for (int e = 0; e < E; e++) d[lane + 64 * e] = (u64)DF<F>::canon(DF<F>::mul(x[e], ninv));
+    }
+    if (more) {
+      if constexpr (PIPE) {
+#pragma unroll
+        for (int e = 0; e < E; e++) x[e] = nx[e];
+      } else {
+        ntt_load<F, E, INV>(data + (poly + stride) * N, lane, x);
+      }
+    }
   }
 }
 
@@ -954,21 +984,42 @@ hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, 
   return hipGetLastError();
 }
 
+template <class F, int E, int WAVES, bool HALF>
+static void launch_ntt_f(int inverse, u64* data, size_t count, const void* tw, u64 ninv,
+                         u32 max_blocks, hipStream_t s) {
+  using W = typename DF<F>::W;
+  constexpr bool PIPE = false;
+  const size_t need = (count + WAVES - 1) / WAVES;
+  const dim3 grid((unsigned)(need < max_blocks ? need : max_blocks)), block(64 * WAVES);
+  if (inverse)
+    hipLaunchKernelGGL((ntt_kernel<F, E, WAVES, HALF, PIPE, true>), grid, block, 0, s, data,
+                       count, tw, (W)ninv);
+  else
+    hipLaunchKernelGGL((ntt_kernel<F, E, WAVES, HALF, PIPE, false>), grid, block, 0, s, data,
+                       count, tw, (W)ninv);
+}
+
+// Launch shapes measured on MI355X (4096 Goldilocks polynomials, kernel us, forward /
+// inverse): N = 1024 one polynomial per wave, 4 waves per workgroup: 22.5 / 25.4 (8 persistent
+// waves per workgroup with the next polynomial prefetched: 23.2 / 26.2, since at two waves
+// per SIMD the VALU, not the loads, sets the pace); N = 2048 with half-size exchange buffers,
+// two 4-wave workgroups per CU: 45.0 / 53.6 (full buffers, one per CU: 62 / 73).
 hipError_t launch_ntt(int is_goldilocks, int inverse, u32 log_n, u64* data, size_t count,
-                      const void* tw, u64 ninv, hipStream_t s) {
-  dim3 grid((unsigned)((count + 3) / 4)), block(256);
+                      const void* tw, u64 ninv, u32 cus, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const u32 all = 0xFFFFFFFFu;
   if (is_goldilocks) {
     if (log_n == 10)
-      hipLaunchKernelGGL((ntt_kernel<Gl, 16>), grid, block, 0, s, data, count, inverse, tw, (u64)ninv);
+      launch_ntt_f<Gl, 16, 4, false>(inverse, data, count, tw, ninv, all, s);
     else if (log_n == 11)
-      hipLaunchKernelGGL((ntt_kernel<Gl, 32>), grid, block, 0, s, data, count, inverse, tw, (u64)ninv);
+      launch_ntt_f<Gl, 32, 4, true>(inverse, data, count, tw, ninv, 2 * cus, s);
     else
       return hipErrorInvalidValue;
   } else {
     if (log_n == 10)
-      hipLaunchKernelGGL((ntt_kernel<Bb, 16>), grid, block, 0, s, data, count, inverse, tw, (u32)ninv);
+      launch_ntt_f<Bb, 16, 4, false>(inverse, data, count, tw, ninv, all, s);
     else if (log_n == 11)
-      hipLaunchKernelGGL((ntt_kernel<Bb, 32>), grid, block, 0, s, data, count, inverse, tw, (u32)ninv);
+      launch_ntt_f<Bb, 32, 4, true>(inverse, data, count, tw, ninv, 6 * cus, s);
     else
       return hipErrorInvalidValue;
   }

@@ -63,8 +63,9 @@ hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_ga
 // Evaluator::not on `count` LWEs of n1 canonical words (out may equal in).
 hipError_t launch_lwe_not(const uint64_t* in, uint64_t* out, uint32_t count, uint32_t n1,
                           uint64_t q, int is_goldilocks, hipStream_t s);
+// Microbench NTT over `count` polynomials on a device with `cus` compute units.
 hipError_t launch_ntt(int is_goldilocks, int inverse, uint32_t log_n, uint64_t* data, size_t count,
-                      const void* tw, uint64_t ninv, hipStream_t s);
+                      const void* tw, uint64_t ninv, uint32_t cus, hipStream_t s);
 // Goldilocks MAC peak microbench: `waves` wavefronts, `iters` x 4 gl_mac2 per lane;
 // out holds waves*64 words.
 hipError_t launch_modmul_peak(uint64_t* out, uint32_t waves, uint32_t iters, uint64_t seed,

@@ -393,7 +393,11 @@ __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0
 // PF: request per-lane twiddles a stage ahead (for one wave per SIMD, where nothing else
 // hides LDS latency); without it they are read just before use, which needs fewer
 // registers (for two waves per SIMD, which hide each other's latency).
-template <class F, int E, bool SMALL = false, bool PF = true>
+// HALF (E = 32): both exchanges run one 16-word group at a time through a buffer of
+// 64*16 + 64 words (half of N + N/16): group v of layout 2 (positions [1024v, 1024v+1024))
+// holds exactly layout 1's words e in [16v, 16v+16) and layout 3's q in [4v, 4v+4), so each
+// group's exchange is self-contained. Less LDS per wave, more waves per CU.
+template <class F, int E, bool SMALL = false, bool PF = true, bool HALF = false>
 __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
                                              const typename DF<F>::W* twv, int lane) {
   using W = typename DF<F>::W;
@@ -416,13 +420,26 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
   // pass 1
   constexpr int S0 = SMALL0 ? 1 : 0;
   pass1_all<F, E, false, S0>(x, std::make_integer_sequence<int, LOGE - S0>{});
+  static_assert(!HALF || V == 2, "half-buffer exchanges are for two 16-word groups");
+  if constexpr (HALF) {
 #pragma unroll
-  for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
-  wave_sync();
+    for (int v = 0; v < V; v++) {
 #pragma unroll
-  for (int v = 0; v < V; v++) {
+      for (int e = 0; e < 16; e++) lds[b1 + 68 * e] = x[16 * v + e];
+      wave_sync();
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
+      for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 4 * k];
+      wave_sync();
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
+    }
   }
   // pass 2: block b = (lane + 64v) >> 2, stage s uses twv[(E << s) + (b << s) + j], j < 2^s
   W t2[V][4][8];
@@ -453,16 +470,31 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
       if constexpr (PF) sched_fence();
     }
   }
+  if constexpr (HALF) {
 #pragma unroll
-  for (int v = 0; v < V; v++) {
+    for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
-  }
-  wave_sync();
+      for (int k = 0; k < 16; k++) lds[b2 + 4 * k] = x[16 * v + k];
+      wave_sync();
 #pragma unroll
-  for (int q = 0; q < E / 4; q++) {
+      for (int q = 0; q < 4; q++) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) x[4 * q + r] = lds[b3 + 272 * q + r];
+        for (int r = 0; r < 4; r++) x[16 * v + 4 * q + r] = lds[b3 + 272 * q + r];
+      }
+      if (v + 1 < V) wave_sync();
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) x[4 * q + r] = lds[b3 + 272 * q + r];
+    }
   }
   // pass 3 twiddles, requested behind the exchange reads (PF) or per group
   W t3[E / 4][3];
@@ -489,7 +521,7 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
 // Inverse transform without the 1/N factor. In: slot layout of ntt_fwd_wave.
 // Out: x[e] = position lane + 64e, canonical. itwv = psi^-bitrev(k) for passes 3 and 2
 // (normally an LDS copy); pass 1 uses the compile-time twiddles.
-template <class F, int E, bool PF = true>
+template <class F, int E, bool PF = true, bool HALF = false>
 __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
                                              const typename DF<F>::W* itwv, int lane) {
   using W = typename DF<F>::W;
@@ -517,16 +549,32 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
     Bfly<F>::gs2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
   }
+  static_assert(!HALF || V == 2, "half-buffer exchanges are for two 16-word groups");
+  if constexpr (HALF) {  // group v: layout 3's q in [4v, 4v+4) -> layout 2's group v
 #pragma unroll
-  for (int q = 0; q < E / 4; q++) {
+    for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) lds[b3 + 272 * q + r] = x[4 * q + r];
-  }
-  wave_sync();
+      for (int q = 0; q < 4; q++) {
 #pragma unroll
-  for (int v = 0; v < V; v++) {
+        for (int r = 0; r < 4; r++) lds[b3 + 272 * q + r] = x[16 * v + 4 * q + r];
+      }
+      wave_sync();
 #pragma unroll
-    for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
+      for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 4 * k];
+      if (v + 1 < V) wave_sync();
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < E / 4; q++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) lds[b3 + 272 * q + r] = x[4 * q + r];
+    }
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[16 * v + k] = lds[b2 + 1088 * v + 4 * k];
+    }
   }
   // pass 2, stages 3..0; stage s uses itwv[(E << s) + (b << s) + j], j < 2^s
   W t2[V][4][8];
@@ -563,15 +611,27 @@ __device__ __forceinline__ void ntt_inv_wave(typename DF<F>::W (&x)[E], typename
       if constexpr (PF) sched_fence();
     }
   }
+  if constexpr (HALF) {  // group v -> layout 1's e in [16v, 16v+16)
 #pragma unroll
-  for (int v = 0; v < V; v++) {
+    for (int v = 0; v < V; v++) {
 #pragma unroll
-    for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
+      for (int k = 0; k < 16; k++) lds[b2 + 4 * k] = x[16 * v + k];
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < 16; e++) x[16 * v + e] = lds[b1 + 68 * e];
+      wave_sync();
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) lds[b2 + 1088 * v + 4 * k] = x[16 * v + k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < E; e++) x[e] = lds[b1 + 68 * e];
+    wave_sync();
   }
-  wave_sync();
-#pragma unroll
-  for (int e = 0; e < E; e++) x[e] = lds[b1 + 68 * e];
-  wave_sync();
   // pass 1
   pass1_all<F, E, true, 0>(x, std::make_integer_sequence<int, LOGE>{});
 }
