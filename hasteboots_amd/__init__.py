@@ -18,6 +18,7 @@ from .tfhe import (  # noqa: F401
     LWECiphertext,
     Parameters,
     SecretKeyPack,
+    TRACE_PARTS,
     device_count,
     last_kernel_ms,
     LATENCY_DEFAULT,

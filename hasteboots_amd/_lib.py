@@ -112,6 +112,9 @@ _PROTOS = {
     "hb_trace_sizes": (C.c_int, [C.c_int, u64p, u64p]),
     "hb_trace_expand": (C.c_int, [C.c_void_p, C.c_int, u64p, u64p, u64p, u64p, u32p, C.c_uint32,
                                   u64p, u64p]),
+    "hb_trace_part_words": (C.c_int, [C.c_int, C.c_int, u64p]),
+    "hb_trace_part": (C.c_int, [C.c_void_p, C.c_int, u64p, u64p, u64p, u64p, C.c_int,
+                                C.c_uint32, u64p]),
     "hb_mixed_batch": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t]),
     "hb_mixed_batch_dev": (C.c_int, [C.POINTER(HbBatchPart), C.c_size_t, C.c_void_p]),
     "hb_ntt_dev": (C.c_int, [C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t, C.c_void_p]),

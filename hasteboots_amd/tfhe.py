@@ -167,6 +167,13 @@ class Decryptor:
         return int(bit.value), int(noise.value)
 
 
+# Finalized PBSTrace containers, in hb.h's HB_TP_* order.
+TRACE_PARTS = ("initial_acc", "final_acc", "input_acc", "output_acc", "input_acc_permuted",
+               "monomial", "monomial_representation", "monomial_times_acc",
+               "external_product_input", "hadamard", "sum_prod", "permutation_info",
+               "modulus_switching")
+
+
 class Evaluator:
     """Evaluator<Q> (tfhe/src/bfhe/evaluate.rs:63-263) backed by keys resident in HBM."""
 
@@ -331,6 +338,30 @@ class Evaluator:
                                     rounds.ctypes.data_as(C.POINTER(C.c_uint32)), rounds.size,
                                     ptr(rec), ptr(tail)))
         return rec, tail
+
+    def trace_part(self, gate: str, c0, c1, core, part, index: int = 0, c2=None) -> np.ndarray:
+        """One finalized PBSTrace container (a name of TRACE_PARTS or its id) of one gate,
+        built from its core trace; shape (planes, rows, N), or (planes, rows) for the
+        monomial representation, flat for permutation_info and modulus_switching."""
+        part_id = TRACE_PARTS.index(part) if isinstance(part, str) else int(part)
+        w = C.c_uint64()
+        check(lib().hb_trace_part_words(self.params.set_id, part_id, C.byref(w)))
+        out = np.zeros(w.value, dtype=np.uint64)
+        core = np.ascontiguousarray(core, np.uint64)
+        c0 = np.ascontiguousarray(c0, np.uint64)
+        c1 = None if c1 is None else np.ascontiguousarray(c1, np.uint64)
+        c2 = None if c2 is None else np.ascontiguousarray(c2, np.uint64)
+        gate_id = 7 if gate == "bootstrap" else GATES[gate]
+        check(lib().hb_trace_part(self._h, gate_id, ptr(c0), ptr(c1), ptr(c2), ptr(core),
+                                  part_id, index, ptr(out)))
+        N = self.params.N
+        name = TRACE_PARTS[part_id]
+        if name in ("permutation_info", "modulus_switching"):
+            return out
+        if name == "monomial_representation":
+            return out.reshape(2, -1)
+        planes = {"monomial": 2, "hadamard": 6}.get(name, 4)
+        return out.reshape(planes, -1, N)
 
 
 class BatchGraph:

@@ -243,6 +243,45 @@ int hb_trace_expand(const hb_ek_t* ek, int gate, const uint64_t* c0, const uint6
                     const uint64_t* c2, const uint64_t* core_trace, const uint32_t* rounds,
                     uint32_t n_rounds, uint64_t* rec, uint64_t* tail);
 
+/* Finalized PBSTrace containers: what the prover takes after
+ * ModulusSwitchingTrace::finalize(n + 1) and BlindRotationTrace::finalize(n)
+ * (vfhe/examples/zk_binary_pbs/zk_nand_goldilocks.rs:97-105). Rows are the n blind-rotation
+ * rounds zero padded to R = 2^ceil(log2 n) (1024); words are canonical field values; each
+ * plane is rows x N, row-major, planes in the struct's field order:
+ *   RLWETrace [poly.0, poly.1, ntt.0, ntt.1]         (basic_ops/rlwe_trace.rs:55-60)
+ *   PolynomialTrace [poly, ntt]                      (basic_ops/rlwe_trace.rs:10-15)
+ *   MonomialTrace [degree, coefficient] (R each)     (basic_ops/rlwe_trace.rs:32-38)
+ *   HadamardTrace [bit.poly, bit.ntt, rlwe.poly.0, rlwe.poly.1, rlwe.ntt.0, rlwe.ntt.1]
+ *                                                    (basic_ops/hadamard_trace.rs:316-342)
+ * AccTrace (trace/src/acc_trace.rs:15-50, finalize :157-202): */
+#define HB_TP_INITIAL_ACC 0            /* RLWETrace, one row (4N) */
+#define HB_TP_FINAL_ACC 1              /* RLWETrace, one row (4N) */
+#define HB_TP_INPUT_ACC 2              /* RLWETrace (4RN) */
+#define HB_TP_OUTPUT_ACC 3             /* RLWETrace (4RN) */
+#define HB_TP_INPUT_ACC_PERMUTED 4     /* output_acc with its poly planes' last row
+                                          += initial - final; ntt planes as output_acc's */
+#define HB_TP_MONOMIAL 5               /* PolynomialTrace (2RN) */
+#define HB_TP_MONOMIAL_REPR 6          /* MonomialTrace (2R) */
+#define HB_TP_MONOMIAL_TIMES_ACC 7     /* RLWETrace (4RN) */
+#define HB_TP_EXTERNAL_PRODUCT_INPUT 8 /* RLWETrace (4RN) */
+/* SumHadamardTrace (basic_ops/hadamard_trace.rs:329-401): */
+#define HB_TP_HADAMARD 9               /* HadamardTrace `index` = half * br_len + digit (6RN) */
+#define HB_TP_SUM_PROD 10              /* RLWETrace (4RN) */
+#define HB_TP_PERMUTATION 11           /* AccTrace::permutation_info.permutation_table (R):
+                                          new_rotation_left(log R, n, 1), row_perm_trace.rs:41-52 */
+#define HB_TP_MODULUS_SWITCH 12        /* ModulusSwitchingTrace (modulus_switching_trace.rs:
+                                          25-49, 77-133): input, output, output_witness,
+                                          helper (M = 2^ceil(log2(n+1)) each), then
+                                          modulus_after, blk_param, helper_upper_bound */
+/* Words of container `part` for parameter set `set`. */
+int hb_trace_part_words(int set, int part, uint64_t* words);
+/* Writes container `part` (hadamard `index`, else ignored) of one gate, built from its core
+ * trace and inputs as in hb_trace_expand, into out (hb_trace_part_words words). The
+ * per-round key consistency check is hb_trace_expand's; this call trusts the core trace. */
+int hb_trace_part(const hb_ek_t* ek, int gate, const uint64_t* c0, const uint64_t* c1,
+                  const uint64_t* c2, const uint64_t* core_trace, int part, uint32_t index,
+                  uint64_t* out);
+
 /* ---- Serving loop (no reference counterpart: the reference evaluates one gate per call).
  * A server admits boolean circuits from many clients (each with its own hb_ek_t) as they
  * arrive, keeps their wires in HBM between layers, admits by an HBM budget (a device

@@ -32,6 +32,12 @@ def build() -> Path:
 
 _lib = None
 
+# Finalized PBSTrace container ids (the order of orc_pbs_trace_part's part argument).
+TRACE_PARTS = ("initial_acc", "final_acc", "input_acc", "output_acc", "input_acc_permuted",
+               "monomial", "monomial_representation", "monomial_times_acc",
+               "external_product_input", "hadamard", "sum_prod", "permutation_info",
+               "modulus_switching")
+
 
 def lib() -> C.CDLL:
     global _lib
@@ -63,6 +69,8 @@ def lib() -> C.CDLL:
         l.orc_trace_sizes.argtypes = [C.POINTER(Params), u64p, u64p]
         l.orc_bootstrap_traced.argtypes = [C.c_void_p, u64p, u64p, u64p, u32p, C.c_uint32, u64p,
                                            u64p]
+        l.orc_pbs_trace_part_words.argtypes = [C.POINTER(Params), C.c_int, u64p]
+        l.orc_pbs_trace_part.argtypes = [C.c_void_p, u64p, u64p, C.c_int, C.c_uint32, u64p]
         l.orc_raw_neg_events.argtypes = [C.c_int]
         l.orc_raw_neg_events.restype = C.c_uint64
         _lib = l
@@ -210,6 +218,16 @@ class OracleKeys:
                                           rounds.ctypes.data_as(u32p), rounds.size, _p(rec),
                                           _p(tail)) == 0
         return out, rec, tail
+
+    def pbs_trace_part(self, c, lut, part: int, index: int = 0) -> np.ndarray:
+        """One finalized PBSTrace container (TRACE_PARTS) of the traced bootstrap of c."""
+        c = np.ascontiguousarray(c, np.uint64)
+        lut = np.ascontiguousarray(lut, np.uint64)
+        w = C.c_uint64()
+        assert lib().orc_pbs_trace_part_words(C.byref(self.p), part, C.cast(C.byref(w), u64p)) == 0
+        out = np.zeros(int(w.value), dtype=np.uint64)
+        assert lib().orc_pbs_trace_part(self.h, _p(c), _p(lut), part, index, _p(out)) == 0
+        return out
 
     def blind_rotate(self, ms, lut) -> np.ndarray:
         ms = np.ascontiguousarray(ms, np.uint32)

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -449,15 +450,18 @@ static void gadget_mac_traced(const typename F::word* src, const typename F::wor
 }
 
 // rounds: sorted indices of the rounds to record into rec (one round record each).
+// sink, when given, also receives every round's record (round index, record).
+using RoundSink = std::function<void(u32, const u64*)>;
 template <class F>
 static void bootstrap_traced(const Params& p, const typename F::word* bsk,
                              const typename F::word* ksk, const typename F::word* c,
                              const typename F::word* lut, typename F::word* out,
                              const u32* rounds, u32 n_rounds, u64* rec, u64* tail, Scratch<F>& s,
-                             const NttTable<F>& T) {
+                             const NttTable<F>& T, const RoundSink* sink = nullptr) {
   using W = typename F::word;
   const u32 N = p.N, n = p.n;
   const size_t R = trace_round_words(p);
+  std::vector<u64> every(sink ? R : 0);
   // lwe_modulus_switch_w_trace (fhe_core/src/modulus_switch.rs:41-86)
   std::vector<u32> ms(n + 1);
   for (u32 k = 0; k <= n; k++) {
@@ -483,6 +487,7 @@ static void bootstrap_traced(const Params& p, const typename F::word* bsk,
   u32 ri = 0;
   for (u32 i = 0; i < n; i++) {
     u64* r = (ri < n_rounds && rounds[ri] == i) ? rec + (size_t)(ri++) * R : nullptr;
+    if (!r && sink) r = every.data();
     if (r) put_rlwe<F>(r, s.acc_a.data(), s.acc_b.data(), N, T);
     // product = ACC * X^a_i via NTT (binary.rs:158-178)
     std::copy(s.acc_a.begin(), s.acc_a.end(), na.begin());
@@ -532,6 +537,7 @@ static void bootstrap_traced(const Params& p, const typename F::word* bsk,
       s.acc_b[k] = F::add(s.acc_b[k], s.ext_b[k]);
     }
     if (r) put_rlwe<F>(r + 4 * N, s.acc_a.data(), s.acc_b.data(), N, T);
+    if (sink) (*sink)(i, r);
   }
   // acc.b[0] += Q/8 (vfhe/src/bfhe/evaluate.rs:91) and the traced key switch.
   W b0 = F::add(s.acc_b[0], F::from_value(p.q >> 3));
@@ -704,6 +710,169 @@ static void run_gates(const Ek<F>& e, int gate, size_t count, const u64* c0s, co
     for (int t = 0; t < threads; t++) pool.emplace_back(worker);
     for (auto& th : pool) th.join();
   }
+}
+
+// ---------------------------------------------------------------------------
+// Finalized PBSTrace containers, as the prover consumes them after
+// ModulusSwitchingTrace::finalize(n + 1) and BlindRotationTrace::finalize(n)
+// (vfhe/examples/zk_binary_pbs/zk_nand_goldilocks.rs:97-105). Restated from:
+//   the append order of blind_rotate_w_trace (fhe_core/src/blind_rotation/rlwe/binary.rs:150-200)
+//     into AccTrace (trace/src/acc_trace.rs:124-155) and SumHadamardTrace
+//     (trace/src/basic_ops/hadamard_trace.rs:329-394);
+//   AccTrace::finalize (trace/src/acc_trace.rs:157-202): input_acc_permuted is a clone of
+//     output_acc whose poly planes' last row gets += initial - final (its ntt planes stay
+//     output_acc's), then every container is zero padded to 2^log_num_round rows and
+//     permutation_info = PermutationInfo::new_rotation_left(log_num_round, n, 1)
+//     (trace/src/basic_ops/row_perm_trace.rs:41-52);
+//   RLWETrace / PolynomialTrace / MonomialTrace::finalize (basic_ops/rlwe_trace.rs:103-109,
+//     129-135, 184-192): padding only when the row count is not a power of two, by
+//     (2^log_num - num) rows;
+//   HadamardTrace / SumHadamardTrace::finalize (basic_ops/hadamard_trace.rs:339-342, 396-401);
+//   ModulusSwitchingTrace::new / finalize (trace/src/modulus_switching_trace.rs:77-133).
+// Words are canonical field values. Plane order follows each struct's field order:
+//   RLWETrace [poly.0, poly.1, ntt.0, ntt.1], PolynomialTrace [poly, ntt],
+//   MonomialTrace [degree, coefficient], HadamardTrace [bit.poly, bit.ntt, rlwe.poly.0,
+//   rlwe.poly.1, rlwe.ntt.0, rlwe.ntt.1]; each plane is rows x N, row-major.
+// ---------------------------------------------------------------------------
+enum TracePart {
+  TP_INITIAL_ACC = 0, TP_FINAL_ACC = 1, TP_INPUT_ACC = 2, TP_OUTPUT_ACC = 3,
+  TP_INPUT_ACC_PERMUTED = 4, TP_MONOMIAL = 5, TP_MONOMIAL_REPR = 6, TP_MONOMIAL_TIMES_ACC = 7,
+  TP_EXTERNAL_PRODUCT_INPUT = 8, TP_HADAMARD = 9, TP_SUM_PROD = 10, TP_PERMUTATION = 11,
+  TP_MODULUS_SWITCH = 12, TP_COUNT = 13
+};
+
+static bool is_pow2(size_t x) { return x && !(x & (x - 1)); }
+
+// One vector-of-rows container with the trace crate's append/finalize rules.
+struct Planes {
+  u32 width;                       // words per row (N, or 1 for MonomialTrace)
+  u32 log_num;                     // rows after padding = 2^log_num
+  std::vector<std::vector<u64>> v;  // one vector per plane
+  Planes(u32 planes, u32 w, u32 lg) : width(w), log_num(lg), v(planes) {}
+  void append(u32 plane, const u64* src) { v[plane].insert(v[plane].end(), src, src + width); }
+  void finalize(size_t num) {
+    if (is_pow2(num)) return;
+    const size_t zeros = (((size_t)1 << log_num) - num) * width;
+    for (auto& x : v) x.resize(x.size() + zeros, 0);
+  }
+  u64* write(u64* dst) const {
+    for (const auto& x : v) dst = std::copy(x.begin(), x.end(), dst);
+    return dst;
+  }
+};
+
+static size_t pbs_part_words(const Params& p, int part) {
+  const size_t N = p.N, R = (size_t)1 << log2u(p.n), M = (size_t)1 << log2u(p.n + 1);
+  switch (part) {
+    case TP_INITIAL_ACC: case TP_FINAL_ACC: return 4 * N;
+    case TP_INPUT_ACC: case TP_OUTPUT_ACC: case TP_INPUT_ACC_PERMUTED:
+    case TP_MONOMIAL_TIMES_ACC: case TP_EXTERNAL_PRODUCT_INPUT: case TP_SUM_PROD:
+      return 4 * R * N;
+    case TP_MONOMIAL: return 2 * R * N;
+    case TP_MONOMIAL_REPR: return 2 * R;
+    case TP_HADAMARD: return 6 * R * N;
+    case TP_PERMUTATION: return R;
+    case TP_MODULUS_SWITCH: return 4 * M + 3;
+    default: return 0;
+  }
+}
+
+// ModulusSwitchingTrace::new + finalize(n + 1) on the recorded input / output
+// (modulus_switching_trace.rs:77-133). Field division by 2q and q is exact since 2q | Q-1.
+static void ms_finalize(const Params& p, const u64* ms_in, const u64* ms_out, u64* out) {
+  const u64 Q = p.q, q2 = 2ull * p.N;
+  const u32 num = p.n + 1, lg = log2u(num);
+  const u64 minus_one = Q - 1;
+  const u64 k = mulmod(minus_one, powmod(2 * q2 % Q, Q - 2, Q), Q);  // (-1) / (q + q)
+  const u64 two_k = mulmod(minus_one, powmod(q2 % Q, Q - 2, Q), Q);  // (-1) / q
+  auto add = [Q](u64 a, u64 b) { return (u64)(((u128)a + b) % Q); };
+  auto sub = [Q](u64 a, u64 b) { return a >= b ? a - b : (u64)((u128)a + Q - b); };
+  const size_t M = (size_t)1 << lg;
+  std::vector<u64> in(ms_in, ms_in + num), o(ms_out, ms_out + num), wit(num), helper(num);
+  for (u32 i = 0; i < num; i++) {
+    wit[i] = o[i] == 0 ? q2 : o[i];                        // b' in [1, q]
+    const u64 two_b_minus_1 = sub(add(wit[i], wit[i]), 1);
+    helper[i] = sub(sub(in[i], mulmod(two_b_minus_1, k, Q)), 1);  // e = a - (2b'-1)k - 1
+  }
+  in.resize(M, 0);
+  o.resize(M, 0);
+  wit.resize(M, q2);
+  helper.resize(M, k);
+  u64* d = out;
+  for (const auto* x : {&in, &o, &wit, &helper}) d = std::copy(x->begin(), x->end(), d);
+  d[0] = q2;
+  d[1] = k;
+  d[2] = two_k;
+}
+
+// Record offsets inside one traced round record (see the traced-bootstrap layout above).
+template <class F>
+static void pbs_trace_part(const Params& p, const typename F::word* bsk,
+                           const typename F::word* ksk, const typename F::word* c,
+                           const typename F::word* lut, int part, u32 index, u64* out,
+                           Scratch<F>& s, const NttTable<F>& T) {
+  const u32 N = p.N, n = p.n, L = p.br_len, lg = log2u(n);
+  std::vector<typename F::word> o(n + 1);
+  if (part == TP_PERMUTATION) {  // PermutationInfo::new_rotation_left(log_num_round, n, 1)
+    std::vector<u64> table((size_t)1 << lg);
+    for (size_t i = 0; i < table.size(); i++) table[i] = i;
+    std::rotate(table.begin(), table.begin() + 1, table.begin() + n);
+    std::copy(table.begin(), table.end(), out);
+    return;
+  }
+  std::vector<u64> tail(part == TP_MODULUS_SWITCH ? trace_tail_words(p) : 0);
+  if (part == TP_MODULUS_SWITCH) {
+    bootstrap_traced<F>(p, bsk, ksk, c, lut, o.data(), nullptr, 0, nullptr, tail.data(), s, T);
+    ms_finalize(p, tail.data(), tail.data() + n + 1, out);
+    return;
+  }
+  const u32 planes = part == TP_HADAMARD ? 6 : part == TP_MONOMIAL || part == TP_MONOMIAL_REPR ? 2 : 4;
+  Planes acc(planes, part == TP_MONOMIAL_REPR ? 1 : N, lg);
+  Planes initial(4, N, 0), final_acc(4, N, 0);
+  const size_t had_off = 18 * (size_t)N + 2, sp_off = (size_t)(18 + 12 * L) * N + 2;
+  auto rlwe4 = [&](Planes& dst, const u64* r4) {
+    for (u32 k = 0; k < 4; k++) dst.append(k, r4 + (size_t)k * N);
+  };
+  RoundSink sink = [&](u32 i, const u64* r) {
+    if (i == 0) rlwe4(initial, r);              // append_acc_initial
+    if (i + 1 == n) rlwe4(final_acc, r + 4 * N);  // append_acc_output
+    switch (part) {
+      case TP_INPUT_ACC: rlwe4(acc, r); break;
+      case TP_OUTPUT_ACC: case TP_INPUT_ACC_PERMUTED: rlwe4(acc, r + 4 * N); break;
+      case TP_MONOMIAL: acc.append(0, r + 8 * N), acc.append(1, r + 9 * N); break;
+      case TP_MONOMIAL_REPR: acc.append(0, r + 10 * N), acc.append(1, r + 10 * N + 1); break;
+      case TP_MONOMIAL_TIMES_ACC: rlwe4(acc, r + 10 * N + 2); break;
+      case TP_EXTERNAL_PRODUCT_INPUT: rlwe4(acc, r + 14 * N + 2); break;
+      case TP_HADAMARD: {  // record [bit, NTT bit, key NTT a, b, key poly a, b]
+        const u64* h = r + had_off + (size_t)6 * index * N;
+        const u32 order[6] = {0, 1, 4, 5, 2, 3};
+        for (u32 k = 0; k < 6; k++) acc.append(k, h + (size_t)order[k] * N);
+        break;
+      }
+      case TP_SUM_PROD: {  // record [NTT a, NTT b, poly a, poly b]
+        const u64* sp = r + sp_off;
+        const u32 order[4] = {2, 3, 0, 1};
+        for (u32 k = 0; k < 4; k++) acc.append(k, sp + (size_t)order[k] * N);
+        break;
+      }
+      default: break;
+    }
+  };
+  bootstrap_traced<F>(p, bsk, ksk, c, lut, o.data(), nullptr, 0, nullptr, nullptr, s, T, &sink);
+  if (part == TP_INPUT_ACC_PERMUTED) {  // last row of the poly planes += initial - final
+    const size_t last = (size_t)(n - 1) * N;
+    for (u32 h = 0; h < 2; h++)
+      for (u32 k = 0; k < N; k++) {
+        u64& a = acc.v[h][last + k];
+        const typename F::word d =
+            F::sub(F::from_value(initial.v[h][k]), F::from_value(final_acc.v[h][k]));
+        a = F::value(F::add(F::from_value(a), d));
+      }
+  }
+  if (part == TP_INITIAL_ACC) { initial.write(out); return; }   // finalize(1): no padding
+  if (part == TP_FINAL_ACC) { final_acc.write(out); return; }
+  acc.finalize(n);
+  acc.write(out);
 }
 
 }  // namespace orc
@@ -944,6 +1113,37 @@ int orc_bootstrap_traced(void* ek, const u64* c, const u64* lut, u64* out, const
     bootstrap_traced<Bb>(p, e->bsk.data(), e->ksk.data(), cc.data(), l.data(), o.data(), rounds,
                          n_rounds, rec, tail, s, table<Bb>(log2u(N)));
     for (u32 k = 0; k < n1; k++) out[k] = Bb::value(o[k]);
+  }
+  return 0;
+}
+
+// Words of one finalized PBSTrace container (part ids in the container comment above);
+// 0 for an unknown part.
+int orc_pbs_trace_part_words(const Params* p, int part, u64* words) {
+  *words = pbs_part_words(*p, part);
+  return *words ? 0 : -1;
+}
+
+// One finalized PBSTrace container of the traced bootstrap of c with lut. index selects
+// the hadamard (0 <= index < 2*br_len: digit index % br_len of half index / br_len).
+int orc_pbs_trace_part(void* ek, const u64* c, const u64* lut, int part, u32 index, u64* out) {
+  EkBase* b = static_cast<EkBase*>(ek);
+  const Params& p = b->p;
+  const u32 N = p.N, n1 = p.n + 1;
+  if (!pbs_part_words(p, part) || (part == TP_HADAMARD && index >= 2 * p.br_len)) return -1;
+  if (p.is_goldilocks) {
+    auto* e = static_cast<Ek<Gl>*>(b);
+    Scratch<Gl> s(N);
+    pbs_trace_part<Gl>(p, e->bsk.data(), e->ksk.data(), c, lut, part, index, out, s,
+                       table<Gl>(log2u(N)));
+  } else {
+    auto* e = static_cast<Ek<Bb>*>(b);
+    Scratch<Bb> s(N);
+    std::vector<u32> cc(n1), l(N);
+    for (u32 k = 0; k < n1; k++) cc[k] = Bb::from_value(c[k]);
+    for (u32 k = 0; k < N; k++) l[k] = Bb::from_value(lut[k]);
+    pbs_trace_part<Bb>(p, e->bsk.data(), e->ksk.data(), cc.data(), l.data(), part, index, out,
+                       s, table<Bb>(log2u(N)));
   }
   return 0;
 }

@@ -1,7 +1,9 @@
 """Traced bootstrapping on the GPU: the core trace emitted by the kernel, expanded by the
 library, equals the oracle's trace records (restating the reference's trace crate) word for
 word; outputs follow the traced path's own semantics."""
+import os
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
@@ -75,3 +77,34 @@ def test_traced_semantics_on_raw_word_keys(hb, orc):
         assert np.array_equal(rec, ref_rec)
         assert np.array_equal(tail, ref_tail)
     assert differ == 3
+
+
+@pytest.mark.parametrize("set_id", [1, 0], ids=["goldilocks", "babybear"])
+def test_finalized_containers_match_oracle(hb, orc, keyset, set_id):
+    """Every finalized PBSTrace container (AccTrace, SumHadamardTrace with all 2*br_len
+    hadamards, permutation_info, ModulusSwitchingTrace), built by hb_trace_part from the
+    GPU's core trace, equals the oracle's restatement of the trace crate's append + finalize
+    word for word (trace/src/acc_trace.rs:157-202, modulus_switching_trace.rs:112-133)."""
+    say(f"finalized containers set={set_id}")
+    P, sk, bsk, ksk = keyset(set_id)
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+    enc = hb.Encryptor(sk)
+    c0, c1 = enc.encrypt_many([1]), enc.encrypt_many([0])
+    outs, core = ev.gate_batch_traced("nand", c0, c1)
+    c = ((c0[0].astype(object) + c1[0].astype(object)) % P.q).astype(np.uint64)
+    lut = orc.build_lut(orc.params(set_id), 0)
+    ok = orc.OracleKeys(set_id, bsk, ksk)
+    jobs = [(k, 0) for k, name in enumerate(hb.TRACE_PARTS) if name != "hadamard"]
+    jobs += [(hb.TRACE_PARTS.index("hadamard"), j) for j in range(2 * P.br_len)]
+
+    def run(job):
+        part, j = job
+        got = ev.trace_part("nand", c0[0], c1[0], core[0], part, j).reshape(-1)
+        ref = ok.pbs_trace_part(c, lut, part, j)
+        return job, bool(np.array_equal(got, ref))
+
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as pool:
+        res = dict(pool.map(run, jobs))
+    bad = [(hb.TRACE_PARTS[p], j) for (p, j), good in res.items() if not good]
+    assert not bad, bad
+    say(f"{len(res)} containers equal")
