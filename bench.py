@@ -199,6 +199,58 @@ def single_nand_leg(hb, cpu: bool, reps: int = 5, seed: int = 61) -> dict:
     return out
 
 
+LEGS = ("end_to_end", "single_nand", "mixed_batch", "microbench", "serving", "cpu_baseline")
+
+
+def compose_line(line: dict, legs: dict) -> dict:
+    """The JSON line: the headline fields, then each leg's own result under its own key
+    (None for a skipped leg)."""
+    unknown = set(legs) - set(LEGS)
+    if unknown:
+        raise KeyError(f"unknown legs {sorted(unknown)}")
+    return {**line, **{k: legs.get(k) for k in LEGS}}
+
+
+def serving_leg(hb, P, set_id, ev, enc, dec, args) -> dict | None:
+    """Poisson arrivals of NAND circuits from several clients (own keys each), a fixed
+    number of circuits per arrival rate, swept past the saturation knee."""
+    rates = [float(r) for r in args.serving_rates.split(",") if r.strip()]
+    if not rates:
+        return None
+    from hasteboots_amd.serving import Server, run_load
+    t = time.time()
+    clients = [(ev, enc, dec)]
+    for seed in range(1, args.serving_clients):
+        sk_c = hb.KeyGen.generate_secret_key(P, seed=seed)
+        bsk_c, ksk_c = sk_c.generate_keys_host()
+        clients.append((hb.Evaluator(keys=(bsk_c, ksk_c), params=P, device=0),
+                        hb.Encryptor(sk_c), hb.Decryptor(sk_c)))
+    log(f"serving: {len(clients)} clients with their own keys ready in {time.time() - t:.1f}s")
+    sweep = []
+    for rate in rates:
+        srv = Server(device=0, max_gates_per_launch=args.serving_max_gates)
+        res = run_load(srv, clients, rate, 0.0, np.random.default_rng([7, int(rate * 1000)]),
+                       circuits=args.serving_circuits)
+        st = srv.stats()
+        srv.close()
+        res.update({"gate_p50_ms": round(st["gate_p50_ms"], 1),
+                    "gate_p99_ms": round(st["gate_p99_ms"], 1),
+                    "launches": st["launches"], "batched_launches": st["batched_launches"],
+                    "max_launch_gates": st["max_launch_gates"],
+                    "kernel_busy_ms": round(st["busy_ms"], 1)})
+        log(f"serving rate {rate}/s: {res}")
+        sweep.append(res)
+    best = max(sweep, key=lambda r: r["gates_per_s"])
+    return {"clients": len(clients), "parameter_set": SET_NAMES[set_id],
+            "max_gates_per_launch": args.serving_max_gates,
+            "circuits_per_rate": args.serving_circuits,
+            "circuit_depth": [1, 64], "circuit_width": [2, 64], "arrivals": "poisson",
+            "latency": "circuit = submit to outputs on host; gate = inputs ready to "
+                       "launch done",
+            "peak_gates_per_s": best["gates_per_s"], "peak_at_rate": best["rate"],
+            "sweep": sweep}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -220,10 +272,13 @@ def main() -> int:
     ap.add_argument("--check", type=int, default=16, help="gates compared with the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--serving-rates", default="1,2,4,8",
+    ap.add_argument("--serving-rates", default="4,8,12,16,20,24,32",
                     help="comma-separated circuit arrival rates (1/s) for the serving leg; "
                          "empty to skip")
-    ap.add_argument("--serving-seconds", type=float, default=8.0)
+    ap.add_argument("--serving-circuits", type=int, default=100,
+                    help="circuits drawn at every serving rate")
+    ap.add_argument("--serving-max-gates", type=int, default=4096,
+                    help="the serving loop's max_gates_per_launch")
     ap.add_argument("--serving-clients", type=int, default=8)
     ap.add_argument("--no-autotune", action="store_true",
                     help="skip the gates-per-workgroup autotuner before the batched leg")
@@ -234,10 +289,11 @@ def main() -> int:
                     help="skip the one-NAND-per-call leg of every parameter set")
     ap.add_argument("--no-micro", action="store_true",
                     help="skip the NTT / external-product / MAC-peak microbenchmarks")
-    ap.add_argument("--only", choices=["micro", "launches"], default=None,
+    ap.add_argument("--only", choices=["micro", "launches", "serving"], default=None,
                     help="run one leg alone and print its JSON: 'micro' the microbenchmarks, "
                          "'launches' --steps eager launches of --batch gates with the default "
-                         "launch rules (the kernel a rocprofv3 pass of that size measures)")
+                         "launch rules (the kernel a rocprofv3 pass of that size measures), "
+                         "'serving' the serving sweep")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before the batched leg, copy a device buffer of this size once "
                          "(a reference kernel for scaling HBM byte counters)")
@@ -280,6 +336,11 @@ def main() -> int:
     bsk, ksk = sk.generate_keys_host()
     ev = hb.Evaluator(keys=(bsk, ksk), params=P, device=0)
     log(f"keygen+upload {time.time() - t:.2f}s ({args.set})")
+
+    if args.only == "serving":
+        serving = serving_leg(hb, P, set_id, ev, hb.Encryptor(sk), hb.Decryptor(sk), args)
+        print(json.dumps({"leg": "serving", "serving": serving}), flush=True)
+        return 0
 
     rng = np.random.default_rng([0, 3])
     b0 = rng.integers(0, 2, size=G)
@@ -403,35 +464,7 @@ def main() -> int:
                  "p99_ms": round(float(np.percentile(lat, 99)), 3), "gates": args.plain}
         log(f"plain: {plain_value:.2f} bootstraps/s, p50 {np.percentile(lat, 50):.1f} ms")
 
-    serving = None
-    rates = [float(r) for r in args.serving_rates.split(",") if r.strip()]
-    if rates:
-        from hasteboots_amd.serving import Server, run_load
-        t = time.time()
-        clients = [(ev, enc, dec)]
-        for seed in range(1, args.serving_clients):
-            sk_c = hb.KeyGen.generate_secret_key(P, seed=seed)
-            bsk_c, ksk_c = sk_c.generate_keys_host()
-            clients.append((hb.Evaluator(keys=(bsk_c, ksk_c), params=P, device=0),
-                            hb.Encryptor(sk_c), hb.Decryptor(sk_c)))
-        log(f"serving: {len(clients)} clients with their own keys ready in {time.time() - t:.1f}s")
-        sweep = []
-        for rate in rates:
-            srv = Server(device=0, max_gates_per_launch=2048)
-            res = run_load(srv, clients, rate, args.serving_seconds,
-                           np.random.default_rng([7, int(rate * 1000)]))
-            st = srv.stats()
-            srv.close()
-            res.update({"gate_p50_ms": round(st["gate_p50_ms"], 1),
-                        "gate_p99_ms": round(st["gate_p99_ms"], 1),
-                        "launches": st["launches"], "kernel_busy_ms": round(st["busy_ms"], 1)})
-            log(f"serving rate {rate}/s: {res}")
-            sweep.append(res)
-        serving = {"clients": len(clients), "parameter_set": SET_NAMES[set_id],
-                   "circuit_depth": [1, 64], "circuit_width": [2, 64], "arrivals": "poisson",
-                   "latency": "circuit = submit to outputs on host; gate = inputs ready to "
-                              "launch done", "sweep": sweep}
-        del clients
+    serving = serving_leg(hb, P, set_id, ev, enc, dec, args)
 
     mixed = None
     if not args.no_mixed:
@@ -520,7 +553,11 @@ def main() -> int:
                      "key_bytes_per_pass": key_bytes, "key_stream_passes": key_passes,
                      "gates_per_key_load": gates_per_key_load, "io_bytes": io_bytes,
                      "one_pass_achieved": round(one_pass / 1e9, 3),
-                     "one_pass_frac": round(one_pass / HBM_PEAK, 6)},
+                     "one_pass_frac": round(one_pass / HBM_PEAK, 6),
+                     "counter_achieved": round(traffic / (kms * 1e-3) / 1e9, 3) if traffic
+                     else None,
+                     "counter_frac": round(traffic / (kms * 1e-3) / HBM_PEAK, 4) if traffic
+                     else None},
         "valu": {"bound": "valu_issue",
                  "achieved": round(valu_rate, 1) if valu_rate else None,
                  "peak": VALU_PEAK, "unit": "wave64 VALU instructions/s",
@@ -534,15 +571,11 @@ def main() -> int:
                  else None,
                  "mac_peak_source": "hb_modmul_peak: register-resident Montgomery MAC pairs, "
                                     "8192 wavefronts"},
-        "end_to_end": e2e,
-        "single_nand": single,
-        "mixed_batch": mixed,
-        "microbench": micro,
         "correct": {"decrypt_mismatches": wrong, "oracle_bit_exact": oracle_ok,
                     "oracle_gates": args.check},
-        "serving": serving,
-        "cpu_baseline": cpu,
     }
+    line = compose_line(line, {"end_to_end": e2e, "single_nand": single, "mixed_batch": mixed,
+                               "microbench": micro, "serving": serving, "cpu_baseline": cpu})
     print(json.dumps(line), flush=True)
     if graph is not None:
         L.hb_batch_graph_destroy(graph)

@@ -662,10 +662,17 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
     if (q.ek->p.N != 1024) return fail(HB_ERR_ARG, "part %zu: unsupported ring dimension", i);
     BootArgs a = make_args(q.ek, q.gate, q.count, q.c0s, q.c1s, q.c2s, nullptr, q.outs,
                            (char*)E.mixed_acc + (size_t)first * per_gate);
-    a.first_block = first;
     first += (uint32_t)q.count;
     args.push_back(a);
   }
+  bool narrow = true;
+  int sets = 0;
+  for (const BootArgs& a : args) narrow = narrow && split8_fits(a.br_len, a.br_bits);
+  for (size_t i : order) sets |= 1 << parts[i].ek->p.set;
+  // Measured, 348 BabyBear + 56 Goldilocks + 574 Zama NANDs: 104.5 ms in latency mode
+  // against 113.4 ms with one wavefront per gate.
+  const MixedKernel kind = mixed_kernel_for(E, sets, first, false);
+  const uint32_t blocks = assign_blocks(args.data(), args.size(), mixed_gpw(kind));
   hipStream_t s = stream ? (hipStream_t)stream : E.stream;
   // the part table and accumulators are free once the previous mixed launch is done
   if (E.mixed_ev_used) HIP_TRY(hipStreamWaitEvent(s, E.mixed_ev, 0));
@@ -674,21 +681,7 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
   int rc = timer_start(s);
   if (rc) return rc;
-  bool narrow = true;
-  int sets = 0;
-  for (const BootArgs& a : args) narrow = narrow && split8_fits(a.br_len, a.br_bits);
-  for (size_t i : order) sets |= 1 << parts[i].ek->p.set;
-  // Latency mode up to split_max gates, and up to one gate per SIMD when the launch mixes
-  // parameter sets: heaviest first, its last workgroup waves hold only the lightest gates.
-  // Measured, 348 BabyBear + 56 Goldilocks + 574 Zama NANDs: 104.5 ms in latency mode
-  // against 113.4 ms with one wavefront per gate.
-  const bool mixes = (sets & (sets - 1)) != 0;
-  const uint32_t cutoff = mixed_split_cutoff(E, sets);
-  if (first <= cutoff || (cutoff > 0 && mixes && first <= E.simd_slots))
-    HIP_TRY(launch_mixed_split(E.mixed_parts, (int)args.size(), first,
-                               mixed_split_waves(narrow), s));
-  else
-    HIP_TRY(launch_mixed(E.mixed_parts, (int)args.size(), first, s));
+  HIP_TRY(launch_mixed_kernel(kind, E.mixed_parts, (int)args.size(), blocks, narrow, s));
   rc = timer_stop(s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(E.mixed_ev, s));

@@ -117,6 +117,30 @@ inline int split_waves(const Engine& E, bool all_goldilocks, size_t count, bool 
 // workgroup fits a CU with 4 waves or with 8: 8 whenever the digit windows allow.
 inline int mixed_split_waves(bool narrow) { return narrow ? 8 : 4; }
 
+// Kernel of a mixed launch of `gates` gates over the parameter sets in `sets` (bit mask):
+// latency mode up to the sets' smallest cutoff (or up to one gate per SIMD when the launch
+// mixes sets: heaviest parts first, the tail workgroups hold the lightest gates), one
+// wavefront per gate up to one gate per SIMD, the (4, 2) batched shape above that.
+// Traced gates have no latency-mode kernel.
+enum class MixedKernel { split, one_wave, batch };
+inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced) {
+  const uint32_t cutoff = mixed_split_cutoff(E, sets);
+  const bool mixes = (sets & (sets - 1)) != 0;
+  if (!traced && (gates <= cutoff || (cutoff > 0 && mixes && gates <= E.simd_slots)))
+    return MixedKernel::split;
+  return gates <= E.simd_slots ? MixedKernel::one_wave : MixedKernel::batch;
+}
+inline uint32_t mixed_gpw(MixedKernel k) { return k == MixedKernel::batch ? kMixedBatchGpw : 1; }
+inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, int nparts,
+                                      uint32_t blocks, bool narrow, hipStream_t s) {
+  switch (k) {
+    case MixedKernel::split:
+      return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), s);
+    case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
+    default: return launch_mixed_batch(parts_dev, nparts, blocks, s);
+  }
+}
+
 Engine& engine();
 int ensure_engine(int device);
 

@@ -726,6 +726,37 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
   }
 }
 
+// Mixed-parameter launch in the batched kernel's throughput shape: GPW gates of one part
+// per workgroup, sharing its twiddle tables, with registers allotted for WPS waves per
+// SIMD. Here first_block counts workgroups, so every part starts on a workgroup boundary;
+// the waves past a part's count leave after the twiddle barrier.
+template <int GPW, int WPS>
+__global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
+void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
+  __shared__ u64 lds[GPW][1024 + 64];
+  __shared__ unsigned short ms[GPW][1024];
+  __shared__ u64 twl[2][1024];
+  const u32 b = blockIdx.x;
+  int p = 0;
+  while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
+  const BootArgs A = parts[p];
+  const int w = threadIdx.x >> 6;
+  const u32 g = (b - A.first_block) * GPW + w;
+  if (A.is_goldilocks) {
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * GPW);
+    if (g >= A.count) return;
+    if (dig8_set(A))
+      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+    else
+      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+  } else {
+    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * GPW);
+    if (g >= A.count) return;
+    bootstrap_gate<Bb>(A, (int)g, reinterpret_cast<u32*>(lds[w]), ms[w], t, t + 1024);
+  }
+}
+
 // Latency-mode mixed launch: one gate per workgroup of WAVES wavefronts, parts as in
 // mixed_kernel (untraced parts only).
 template <int WAVES>
@@ -981,6 +1012,13 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_g
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {
   hipLaunchKernelGGL(mixed_kernel, dim3(total_gates), dim3(64), 0, s, parts_dev, nparts);
+  return hipGetLastError();
+}
+
+hipError_t launch_mixed_batch(const BootArgs* parts_dev, int nparts, u32 total_blocks,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2>), dim3(total_blocks),
+                     dim3(64 * kMixedBatchGpw), 0, s, parts_dev, nparts);
   return hipGetLastError();
 }
 

@@ -60,6 +60,22 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t to
                               int waves, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
+// Throughput-mode mixed launch: kMixedBatchGpw gates of one part per workgroup, two waves
+// per SIMD (the batched kernel's (4, 2) shape). first_block counts workgroups here, so
+// part k occupies ceil(count_k / kMixedBatchGpw) workgroups from its first_block.
+constexpr uint32_t kMixedBatchGpw = 4;
+hipError_t launch_mixed_batch(const BootArgs* parts_dev, int nparts, uint32_t total_blocks,
+                              hipStream_t s);
+// Sets first_block of parts[0..n) for a launch with `gpw` gates per workgroup (1 for
+// launch_mixed and launch_mixed_split) and returns the workgroup count.
+inline uint32_t assign_blocks(BootArgs* parts, size_t n, uint32_t gpw) {
+  uint32_t b = 0;
+  for (size_t k = 0; k < n; k++) {
+    parts[k].first_block = b;
+    b += (parts[k].count + gpw - 1) / gpw;
+  }
+  return b;
+}
 // Evaluator::not on `count` LWEs of n1 canonical words (out may equal in).
 hipError_t launch_lwe_not(const uint64_t* in, uint64_t* out, uint32_t count, uint32_t n1,
                           uint64_t q, int is_goldilocks, hipStream_t s);

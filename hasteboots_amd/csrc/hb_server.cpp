@@ -213,14 +213,30 @@ int complete(hb_server* S, Slot& sl) {
   return HB_OK;
 }
 
+// Gates of the current layer that request r has not launched yet.
+uint32_t layer_backlog(const Request* r) {
+  if (r->in_flight || r->next >= r->n_gates) return 0;
+  size_t ri = std::upper_bound(r->run_end.begin(), r->run_end.end(), r->next) - r->run_end.begin();
+  const uint32_t layer = r->run_layer[ri];
+  while (ri + 1 < r->run_end.size() && r->run_layer[ri + 1] == layer) ri++;
+  return r->run_end[ri] - r->next;
+}
+
 // Gathers the next ready run of each idle request into a launch on slot `sl`.
 int launch(hb_server* S, Slot& sl) {
-  // At most one latency-mode workgroup per CU per launch: under load, many short launches
-  // beat few long ones. Measured on MI355X (8 clients, Goldilocks), cap 256 against 2048
-  // gates: 8 circuits/s p99 2.02 s against 2.62 s (gate p99 44 against 65 ms), 12 circuits/s
-  // p50/p99 3.4/5.3 s against 5.5/10.6 s and 10.4 k against 9.3 k gates/s; unchanged at 4/s.
+  // Latency mode: at most one latency-mode workgroup per CU per launch, since under
+  // moderate load many short launches beat few long ones (measured on MI355X, 8 clients,
+  // Goldilocks, cap 256 against 2048 gates run one wavefront per gate: 8 circuits/s p99
+  // 2.02 s against 2.62 s, 12 circuits/s p50/p99 3.4/5.3 s against 5.5/10.6 s).
+  // Throughput mode: once the ready gates fill the batched (4, 2) shape's resident
+  // workgroups (two per CU), a launch takes up to max_gates of them in that shape.
   const uint32_t cus = std::max<uint32_t>(1, engine().simd_slots / 4);
-  uint32_t cap = std::min(S->max_gates, cus), used = 0, nparts = 0;
+  const uint32_t batch_min = 2 * kMixedBatchGpw * cus;
+  uint32_t ready = 0;
+  if (S->max_gates >= batch_min)
+    for (const Request* r : S->active) ready += layer_backlog(r);
+  const bool throughput = S->max_gates >= batch_min && ready >= batch_min;
+  uint32_t cap = throughput ? S->max_gates : std::min(S->max_gates, cus), used = 0, nparts = 0;
   for (Request* r : S->active) {
     if (r->in_flight || r->next >= r->n_gates || used >= cap) continue;
     // consecutive runs (one kind each) of the current layer, up to the capacity; the
@@ -250,9 +266,6 @@ int launch(hb_server* S, Slot& sl) {
     sl.work.emplace_back(r, take);
   }
   if (used == 0) return HB_OK;
-  HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
-                         hipMemcpyHostToDevice, sl.s));
-  HIP_TRY(hipEventRecord(sl.ev0, sl.s));
   bool any_traced = false, narrow = true;
   int sets = 0;
   for (uint32_t k = 0; k < nparts; k++) {
@@ -261,16 +274,19 @@ int launch(hb_server* S, Slot& sl) {
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
   for (const auto& w : sl.work) sets |= 1 << w.first->ek->p.set;
-  if (!any_traced && used <= mixed_split_cutoff(engine(), sets))
-    HIP_TRY(launch_mixed_split(sl.parts_dev, (int)nparts, used, mixed_split_waves(narrow),
-                               sl.s));
-  else
-    HIP_TRY(launch_mixed(sl.parts_dev, (int)nparts, used, sl.s));
+  const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced);
+  const uint32_t blocks = assign_blocks(sl.parts_host, nparts, mixed_gpw(kind));
+  HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
+                         hipMemcpyHostToDevice, sl.s));
+  HIP_TRY(hipEventRecord(sl.ev0, sl.s));
+  HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sl.s));
   HIP_TRY(hipEventRecord(sl.ev1, sl.s));
   sl.busy = true;
   sl.gates = used;
   sl.seq = ++S->launch_seq;
   S->st.launches++;
+  if (kind == MixedKernel::batch) S->st.batched_launches++;
+  S->st.max_launch_gates = std::max<uint64_t>(S->st.max_launch_gates, used);
   return HB_OK;
 }
 

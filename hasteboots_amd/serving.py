@@ -129,18 +129,22 @@ class Server:
 
 
 def run_load(server: Server, clients, rate: float, duration: float, rng: np.random.Generator,
-             depth=(1, 64), width=(2, 64), verify: bool = True, log=None) -> dict:
-    """Poisson arrivals at `rate` circuits/s for `duration` seconds. Each arrival picks a
-    client (evaluator, encryptor, decryptor) uniformly, a NAND circuit with depth and width
-    uniform in the given inclusive ranges, and uniform input bits. Returns latency
-    percentiles, throughput and the number of outputs that decrypt wrongly."""
+             depth=(1, 64), width=(2, 64), verify: bool = True, log=None,
+             circuits: int | None = None) -> dict:
+    """Poisson arrivals at `rate` circuits/s for `duration` seconds, or of exactly
+    `circuits` circuits when that is given. Each arrival picks a client (evaluator,
+    encryptor, decryptor) uniformly, a NAND circuit with depth and width uniform in the
+    given inclusive ranges, and uniform input bits. Returns latency percentiles (with the
+    sample count), throughput and the number of outputs that decrypt wrongly."""
     t0 = time.perf_counter()
     pending = []
     t_next = t0
     total_gates = 0
     while True:
         t_next += rng.exponential(1.0 / rate)
-        if t_next - t0 > duration:
+        if circuits is None and t_next - t0 > duration:
+            break
+        if circuits is not None and len(pending) >= circuits:
             break
         while time.perf_counter() < t_next:
             time.sleep(min(0.002, max(0.0, t_next - time.perf_counter())))
@@ -163,7 +167,8 @@ def run_load(server: Server, clients, rate: float, duration: float, rng: np.rand
             got = [dec.decrypt_with_noise(out[k])[0] for k in range(out.shape[0])]
             wrong += int(np.sum(np.asarray(got) != expect))
     elapsed = time.perf_counter() - t0
-    res = {"rate": rate, "circuits": len(pending), "gates": total_gates,
+    res = {"rate": rate, "circuits": len(pending), "latency_samples": len(lat),
+           "gates": total_gates,
            "elapsed_s": round(elapsed, 3), "arrival_window_s": round(t_arrivals, 3),
            "gates_per_s": round(total_gates / elapsed, 1) if elapsed > 0 else 0.0,
            "p50_ms": round(float(np.percentile(lat, 50)), 1) if lat else None,

@@ -312,11 +312,16 @@ typedef struct {
   /* gate latency = from a gate's inputs being ready (admission or previous layer done) to
    * the completion of the launch that ran it; gate-weighted percentiles since creation */
   double gate_p50_ms, gate_p99_ms;
+  /* launches run in the throughput shape; largest launch so far (gates) */
+  uint64_t batched_launches, max_launch_gates;
 } hb_server_stats_t;
 
-/* max_gates_per_launch: gate slots per launch (a launch takes at most one gate per compute
- * unit, 256 on MI355X, whichever is smaller); hbm_budget_bytes: arena size (0 = half of the
- * free device memory, at most 32 GiB). */
+/* max_gates_per_launch: gate slots per launch. A launch takes at most one gate per compute
+ * unit (256 on MI355X) in latency mode; when max_gates_per_launch allows it and the ready
+ * gates of the admitted circuits number at least two batched workgroups of 4 per compute
+ * unit (2048 on MI355X), it takes up to max_gates_per_launch gates in the batched kernel's
+ * (4 gates per workgroup, 2 waves per SIMD) shape. hbm_budget_bytes: arena size (0 = half
+ * of the free device memory, at most 32 GiB). */
 int hb_server_create(int device, uint32_t max_gates_per_launch, uint64_t hbm_budget_bytes,
                      hb_server_t** out);
 int hb_server_destroy(hb_server_t* server);

@@ -113,3 +113,27 @@ def test_mixed_geometric_counts_config(hb, orc, evaluators):
         assert np.array_equal(out[pick], ref), s
         alone = ev.nand(hb.LWECiphertext(c0[pick[0]]), hb.LWECiphertext(c1[pick[0]]))
         assert np.array_equal(alone.words, out[pick[0]])
+
+
+def test_mixed_throughput_shape_every_output(hb, orc, keyset):
+    """701 BabyBear + 903 Goldilocks + 301 Zama NANDs (above one gate per SIMD, so the
+    launch takes the batched 4-gates-per-workgroup shape, with parts ending inside a
+    workgroup): every output equals the oracle's."""
+    import os
+    rng = stream(43, "mixed-batch-shape")
+    counts = {hb.SET_BABYBEAR: 701, hb.SET_GOLDILOCKS: 903, hb.SET_ZAMA: 301}
+    parts, data = [], []
+    for s, count in counts.items():
+        P, sk, bsk, ksk = keyset(s)
+        ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+        enc = hb.Encryptor(sk)
+        b0, b1 = rng.integers(0, 2, size=(2, count))
+        c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+        parts.append((ev, "nand", c0, c1))
+        data.append((s, bsk, ksk, c0, c1))
+    outs = hb.mixed_batch(parts)
+    for (s, bsk, ksk, c0, c1), out in zip(data, outs):
+        print(f"oracle set {s}: {out.shape[0]} gates", flush=True)
+        ref = orc.OracleKeys(s, bsk, ksk).gate_batch(0, c0, c1, threads=min(16, os.cpu_count() or 1))
+        bad = np.flatnonzero(~np.all(out == ref, axis=1))
+        assert bad.size == 0, (s, bad[:8])

@@ -191,3 +191,46 @@ def test_config_size_circuits_alone_equal_mixed(hb, orc, clients):
         g = np.arange(16 * layer, 16 * (layer + 1))
         wires[16 + g] = ok.gate_batch(0, wires[circ.in0[g]], wires[circ.in1[g]], threads=16)
     assert np.array_equal(out, wires[circ.outputs])
+
+
+def test_throughput_mode_equals_latency_mode(hb, keyset, clients):
+    """With a backlog of ready gates above two batched workgroups per CU, the server runs
+    launches in the batched (4 gates per workgroup, 2 waves per SIMD) shape over mixed
+    parameter sets and clients; every circuit's outputs, and a traced circuit's core
+    traces, are the same words as when the circuit runs alone in latency mode."""
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    P0, sk0, bsk0, ksk0 = keyset(hb.SET_BABYBEAR)
+    bb = (hb.Evaluator(keys=(bsk0, ksk0), params=P0), hb.Encryptor(sk0), hb.Decryptor(sk0))
+    pool = list(clients) + [bb]
+    rng = stream(57, "throughput")
+    jobs = []
+    for i in range(36):
+        ev, enc, dec = pool[i % len(pool)]
+        circ = random_nand_circuit(rng, 2, 64)
+        bits = rng.integers(0, 2, size=64)
+        jobs.append((ev, dec, circ, bits, enc.encrypt_many(bits)))
+    ev_t, enc_t, dec_t = clients[0]
+    circ_t = random_nand_circuit(rng, 2, 8)
+    bits_t = rng.integers(0, 2, size=8)
+    cts_t = enc_t.encrypt_many(bits_t)
+    srv = Server(max_gates_per_launch=4096)
+    ids = [srv.submit(ev, circ, cts) for ev, dec, circ, bits, cts in jobs]
+    rid_t = srv.submit(ev_t, circ_t, cts_t, traced=True)
+    mixed = [srv.wait(r)[0] for r in ids]
+    out_t, _, core_t = srv.wait(rid_t)
+    st = srv.stats()
+    srv.close()
+    say(f"throughput stats {st}")
+    assert st["batched_launches"] >= 1 and st["max_launch_gates"] >= 2048
+    lat = Server(max_gates_per_launch=256)
+    alone = [lat.wait(lat.submit(ev, circ, cts))[0] for ev, dec, circ, bits, cts in jobs]
+    a_out, _, a_core = lat.wait(lat.submit(ev_t, circ_t, cts_t, traced=True))
+    st_lat = lat.stats()
+    lat.close()
+    assert st_lat["batched_launches"] == 0 and st_lat["max_launch_gates"] <= 256
+    for k, ((ev, dec, circ, bits, cts), a, m) in enumerate(zip(jobs, alone, mixed)):
+        assert np.array_equal(a, m), k
+        got = [dec.decrypt(hb.LWECiphertext(m[j])) for j in range(m.shape[0])]
+        assert got == list(circ.eval_plain(bits)), k
+    assert np.array_equal(out_t, a_out)
+    assert np.array_equal(core_t, a_core)
