@@ -199,7 +199,44 @@ def single_nand_leg(hb, cpu: bool, reps: int = 5, seed: int = 61) -> dict:
     return out
 
 
-LEGS = ("end_to_end", "single_nand", "mixed_batch", "microbench", "serving", "cpu_baseline")
+def keygen_leg(hb, L, check) -> dict:
+    """Evaluation-key generation per parameter set: on the host (hb_keygen_host, up to 16
+    threads) and on the GPU (hb_keygen_dev into device buffers, returns when written;
+    median of 3 after one warm-up), and hb_ek_create end to end (device keygen, device-form
+    conversion, canonical host copy)."""
+    out = {}
+    for set_id, name in SET_NAMES.items():
+        P = hb.Parameters.of(set_id)
+        sk = hb.KeyGen.generate_secret_key(P, seed=0)
+        t = time.perf_counter()
+        bsk, ksk = sk.generate_keys_host()
+        host_s = time.perf_counter() - t
+        bufs = [C.c_void_p(), C.c_void_p()]
+        for b, a in zip(bufs, (bsk, ksk)):
+            check(L.hb_dev_alloc(a.nbytes, C.byref(b)))
+        ts = []
+        for _ in range(4):
+            t = time.perf_counter()
+            check(L.hb_keygen_dev(sk._h, bufs[0], bufs[1], None))
+            ts.append(time.perf_counter() - t)
+        dev = np.zeros_like(ksk)
+        check(L.hb_memcpy_d2h(dev.ctypes.data, bufs[1], dev.nbytes))
+        same_ksk = bool(np.array_equal(dev, ksk))
+        for b in bufs:
+            L.hb_dev_free(b)
+        t = time.perf_counter()
+        ev = hb.Evaluator(sk)
+        create_s = time.perf_counter() - t
+        del ev
+        out[name] = {"host_s": round(host_s, 3), "device_ms": round(1e3 * float(np.median(ts[1:])), 2),
+                     "ek_create_s": round(create_s, 3), "key_bytes": int((bsk.size + ksk.size) * 8),
+                     "ksk_equal": same_ksk}
+        log(f"keygen {name}: {out[name]}")
+    return out
+
+
+LEGS = ("end_to_end", "single_nand", "mixed_batch", "microbench", "serving", "keygen",
+        "cpu_baseline")
 
 
 def compose_line(line: dict, legs: dict) -> dict:
@@ -269,7 +306,7 @@ def main() -> int:
     ap.add_argument("--set", choices=list(SETS), default="goldilocks")
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--plain", type=int, default=8, help="gates timed on the plain path")
-    ap.add_argument("--check", type=int, default=16, help="gates compared with the oracle")
+    ap.add_argument("--check", type=int, default=256, help="gates compared with the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-rates", default="4,8,12,16,20,24,32",
@@ -287,6 +324,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-parameter leg")
     ap.add_argument("--no-single", action="store_true",
                     help="skip the one-NAND-per-call leg of every parameter set")
+    ap.add_argument("--no-keygen", action="store_true",
+                    help="skip the host / device key-generation timings")
     ap.add_argument("--no-micro", action="store_true",
                     help="skip the NTT / external-product / MAC-peak microbenchmarks")
     ap.add_argument("--only", choices=["micro", "launches", "serving"], default=None,
@@ -480,6 +519,8 @@ def main() -> int:
         micro = microbench(L, check, hb)
         log(f"microbench: {micro}")
 
+    keygen = None if args.no_keygen else keygen_leg(hb, L, check)
+
     wb = 8 if P.is_goldilocks else 4
     key_bytes = (bsk.size + ksk.size) * wb
     io_bytes = G * 3 * n1 * 8
@@ -575,7 +616,8 @@ def main() -> int:
                     "oracle_gates": args.check},
     }
     line = compose_line(line, {"end_to_end": e2e, "single_nand": single, "mixed_batch": mixed,
-                               "microbench": micro, "serving": serving, "cpu_baseline": cpu})
+                               "microbench": micro, "serving": serving, "keygen": keygen,
+                               "cpu_baseline": cpu})
     print(json.dumps(line), flush=True)
     if graph is not None:
         L.hb_batch_graph_destroy(graph)

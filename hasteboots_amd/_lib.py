@@ -113,6 +113,16 @@ _PROTOS = {
     "hb_trace_sizes": (C.c_int, [C.c_int, u64p, u64p]),
     "hb_trace_expand": (C.c_int, [C.c_void_p, C.c_int, u64p, u64p, u64p, u64p, u32p, C.c_uint32,
                                   u64p, u64p]),
+    "hb_keygen_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "hb_keys_serialize": (C.c_int, [C.c_int, u64p, u64p, C.c_void_p, C.c_size_t,
+                                    C.POINTER(C.c_size_t)]),
+    "hb_keys_deserialize": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_int), u64p, u64p]),
+    "hb_ek_serialize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "hb_ek_deserialize": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_void_p)]),
+    "hb_lwe_serialize": (C.c_int, [C.c_int, C.c_size_t, u64p, C.c_void_p, C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
+    "hb_lwe_deserialize": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_size_t), u64p, C.c_size_t]),
     "hb_trace_part_words": (C.c_int, [C.c_int, C.c_int, u64p]),
     "hb_trace_part": (C.c_int, [C.c_void_p, C.c_int, u64p, u64p, u64p, u64p, C.c_int,
                                 C.c_uint32, u64p]),

@@ -17,6 +17,7 @@
 #include "hb_internal.hpp"
 #include "hb_kernels.hpp"
 #include "hb_keys.hpp"
+#include "hb_sample.hpp"
 
 using namespace hb;
 
@@ -207,34 +208,87 @@ uint32_t ilog2u(uint32_t x) {
 
 namespace {
 
-int upload_keys(hb_ek* ek) {
+// Makes canonical keys resident in the device form the bootstrap kernels read: bsk_c and
+// ksk_c are device arrays of canonical words, converted on `s` (× N^-1, Montgomery).
+int install_keys(hb_ek* ek, const uint64_t* bsk_c, const uint64_t* ksk_c, hipStream_t s) {
   const ParamSet& p = ek->p;
   const size_t wb = p.is_goldilocks ? 8 : 4;
   const uint64_t nb = bsk_words(p), nk = ksk_words(p);
   HostNtt ntt;
   ntt.init(p, ilog2u(p.N));
-  const uint64_t ninv = ntt.n_inv;
-  auto convert = [&](const std::vector<uint64_t>& src, void** dst) -> int {
-    std::vector<uint8_t> buf(src.size() * wb);
-    if (p.is_goldilocks) {
-      uint64_t* w = reinterpret_cast<uint64_t*>(buf.data());
-      for (size_t k = 0; k < src.size(); k++) w[k] = to_device_word(p, src[k], ninv);
-    } else {
-      uint32_t* w = reinterpret_cast<uint32_t*>(buf.data());
-      for (size_t k = 0; k < src.size(); k++) w[k] = (uint32_t)to_device_word(p, src[k], ninv);
-    }
-    HIP_TRY(hipMalloc(dst, buf.size()));
-    HIP_TRY(hipMemcpy(*dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
-    return HB_OK;
-  };
-  (void)nb;
-  (void)nk;
-  int rc = convert(ek->bsk_host, &ek->bsk_dev);
-  if (rc) return rc;
-  rc = convert(ek->ksk_host, &ek->ksk_dev);
-  if (rc) return rc;
+  HIP_TRY(hipMalloc(&ek->bsk_dev, nb * wb));
+  HIP_TRY(hipMalloc(&ek->ksk_dev, nk * wb));
+  HIP_TRY(launch_keys_to_device(bsk_c, ek->bsk_dev, nb, ntt.n_inv, p.is_goldilocks, p.q, s));
+  HIP_TRY(launch_keys_to_device(ksk_c, ek->ksk_dev, nk, ntt.n_inv, p.is_goldilocks, p.q, s));
   HIP_TRY(hipMalloc(&ek->lut_dev, (size_t)p.N * 8));
   HIP_TRY(hipEventCreateWithFlags(&ek->acc_ev, hipEventDisableTiming));
+  return HB_OK;
+}
+
+// Device scratch freed when the scope ends (after the stream's work on it is done: the
+// callers synchronise first).
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// From the canonical host copies in ek->bsk_host / ksk_host.
+int upload_keys(hb_ek* ek) {
+  Engine& E = engine();
+  DevBuf bsk, ksk;
+  HIP_TRY(hipMalloc(&bsk.p, ek->bsk_host.size() * 8));
+  HIP_TRY(hipMalloc(&ksk.p, ek->ksk_host.size() * 8));
+  HIP_TRY(hipMemcpyAsync(bsk.p, ek->bsk_host.data(), ek->bsk_host.size() * 8,
+                         hipMemcpyHostToDevice, E.stream));
+  HIP_TRY(hipMemcpyAsync(ksk.p, ek->ksk_host.data(), ek->ksk_host.size() * 8,
+                         hipMemcpyHostToDevice, E.stream));
+  int rc = install_keys(ek, static_cast<uint64_t*>(bsk.p), static_cast<uint64_t*>(ksk.p), E.stream);
+  HIP_TRY(hipStreamSynchronize(E.stream));
+  return rc;
+}
+
+// BinaryBlindRotationKey::generate + KeySwitchingRLWEKey::generate on the GPU: the words
+// of generate_bsk / generate_ksk (hb_keys.cpp), canonical layout, into device arrays.
+int keygen_device(const SecretKey& sk, uint64_t* bsk_c, uint64_t* ksk_c, hipStream_t s) {
+  const ParamSet& p = sk.p;
+  Engine& E = engine();
+  const uint32_t N = p.N;
+  if (N != 1024) return fail(HB_ERR_ARG, "device key generation needs N = 1024");
+  HostNtt ntt;
+  ntt.init(p, ilog2u(N));
+  std::vector<uint64_t> lwe_ntt(N, 0);
+  for (uint32_t k = 0; k < p.n; k++) lwe_ntt[k] = sk.lwe_sk[k];
+  ntt.fwd(lwe_ntt.data());
+  std::vector<uint8_t> bits(p.n);
+  for (uint32_t k = 0; k < p.n; k++) bits[k] = (uint8_t)sk.lwe_sk[k];
+  const uint32_t bsk_rows = p.n * 2 * p.br_len;
+  DevBuf err, ring, lwe, sk_bits;
+  HIP_TRY(hipMalloc(&err.p, (size_t)bsk_rows * N * 8));
+  HIP_TRY(hipMalloc(&ring.p, (size_t)N * 8));
+  HIP_TRY(hipMalloc(&lwe.p, (size_t)N * 8));
+  HIP_TRY(hipMalloc(&sk_bits.p, p.n));
+  HIP_TRY(hipMemcpyAsync(ring.p, sk.ring_sk_ntt.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(lwe.p, lwe_ntt.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(sk_bits.p, bits.data(), p.n, hipMemcpyHostToDevice, s));
+  KeygenArgs a{};
+  derive_key(sk.seed, a.key);
+  a.N = N;
+  a.is_goldilocks = p.is_goldilocks;
+  a.q = p.q;
+  a.lwe_sk = static_cast<const uint8_t*>(sk_bits.p);
+  const void* tw = E.tables.tw[p.is_goldilocks ? 1 : 0][0][0];
+  const uint32_t cus = E.simd_slots / 4;
+  a.kind = STREAM_BSK, a.rows = bsk_rows, a.L = p.br_len, a.basis_bits = p.br_bits;
+  a.is_bsk = 1, a.sigma = p.ring_sigma;
+  const uint64_t* ring_ntt = static_cast<const uint64_t*>(ring.p);
+  HIP_TRY(launch_keygen(a, bsk_c, static_cast<uint64_t*>(err.p), ring_ntt, ring_ntt, tw, cus, s));
+  a.kind = STREAM_KSK, a.rows = p.ks_len, a.L = 1, a.basis_bits = p.ks_bits;
+  a.is_bsk = 0, a.sigma = p.ks_sigma;
+  HIP_TRY(launch_keygen(a, ksk_c, static_cast<uint64_t*>(err.p),
+                        static_cast<const uint64_t*>(lwe.p), ring_ntt, tw, cus, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return HB_OK;
 }
 
@@ -493,20 +547,44 @@ int hb_ek_create(const hb_sk_t* sk, int device, hb_ek_t** out) {
   if (!sk || !out) return fail(HB_ERR_ARG, "null argument");
   int rc = ensure_engine(device);
   if (rc) return rc;
-  auto* ek = new (std::nothrow) hb_ek;
+  std::unique_ptr<hb_ek> ek(new (std::nothrow) hb_ek);
   if (!ek) return fail(HB_ERR_NOMEM, "allocation failed");
   ek->p = sk->sk.p;
   ek->bsk_host.resize(bsk_words(ek->p));
   ek->ksk_host.resize(ksk_words(ek->p));
-  generate_bsk(sk->sk, ek->bsk_host.data());
-  generate_ksk(sk->sk, ek->ksk_host.data());
-  rc = upload_keys(ek);
+  Engine& E = engine();
+  auto make = [&]() -> int {
+    DevBuf bsk, ksk;
+    HIP_TRY(hipMalloc(&bsk.p, ek->bsk_host.size() * 8));
+    HIP_TRY(hipMalloc(&ksk.p, ek->ksk_host.size() * 8));
+    uint64_t* b = static_cast<uint64_t*>(bsk.p);
+    uint64_t* k = static_cast<uint64_t*>(ksk.p);
+    int r = keygen_device(sk->sk, b, k, E.stream);
+    if (r) return r;
+    r = install_keys(ek.get(), b, k, E.stream);
+    if (r) return r;
+    HIP_TRY(hipMemcpyAsync(ek->bsk_host.data(), b, ek->bsk_host.size() * 8, hipMemcpyDeviceToHost,
+                           E.stream));
+    HIP_TRY(hipMemcpyAsync(ek->ksk_host.data(), k, ek->ksk_host.size() * 8, hipMemcpyDeviceToHost,
+                           E.stream));
+    HIP_TRY(hipStreamSynchronize(E.stream));
+    return HB_OK;
+  };
+  rc = make();
   if (rc) {
-    hb_ek_destroy(ek);
+    hb_ek_destroy(ek.release());
     return rc;
   }
-  *out = ek;
+  *out = ek.release();
   return HB_OK;
+}
+
+int hb_keygen_dev(const hb_sk_t* sk, uint64_t* bsk_dev, uint64_t* ksk_dev, void* stream) {
+  if (!sk || !bsk_dev || !ksk_dev) return fail(HB_ERR_ARG, "null argument");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : engine().stream;
+  return keygen_device(sk->sk, bsk_dev, ksk_dev, s);
 }
 
 int hb_ek_export(const hb_ek_t* ek, uint64_t* bsk, uint64_t* ksk) {

@@ -76,6 +76,25 @@ inline uint32_t assign_blocks(BootArgs* parts, size_t n, uint32_t gpw) {
   }
   return b;
 }
+// Evaluation-key generation (hb_keygen.hip): `rows` RLWE samples of N coefficients into
+// `keys` (canonical [row][a, b][N] layout), from the counter-based samplers of hb_sample.hpp.
+// BSK (is_bsk): row r = (i * 2 + c) * L + j, stream (kind, i), gadget basis 2^basis_bits,
+// s = NTT(s_ring). KSK: row r, stream (kind, r), s = NTT(s_lwe ‖ 0), plus B_ks^r NTT(s_ring).
+struct KeygenArgs {
+  uint32_t key[8];
+  uint32_t kind, rows, N, L, basis_bits;
+  int is_bsk, is_goldilocks;
+  uint64_t q;
+  double sigma;
+  const uint8_t* lwe_sk;  // device, n bytes (BSK only)
+};
+// err: rows * N words of device scratch; s_ntt and ring_ntt: N canonical words on device.
+hipError_t launch_keygen(const KeygenArgs& a, uint64_t* keys, uint64_t* err, const uint64_t* s_ntt,
+                         const uint64_t* ring_ntt, const void* tw_fwd, uint32_t cus, hipStream_t s);
+// Canonical key words -> the bootstrap kernels' device form (× scale, Montgomery; u32 words
+// for BabyBear).
+hipError_t launch_keys_to_device(const uint64_t* src, void* dst, size_t words, uint64_t scale,
+                                 int is_goldilocks, uint64_t q, hipStream_t s);
 // Evaluator::not on `count` LWEs of n1 canonical words (out may equal in).
 hipError_t launch_lwe_not(const uint64_t* in, uint64_t* out, uint32_t count, uint32_t n1,
                           uint64_t q, int is_goldilocks, hipStream_t s);

@@ -6,47 +6,22 @@
 #include <thread>
 
 #include "hb_field.hpp"
+#include "hb_sample.hpp"
 
 namespace hb {
 
 // ------------------------------ ChaCha20 ------------------------------------
-static inline uint32_t rotl32(uint32_t x, int r) { return (x << r) | (x >> (32 - r)); }
-static inline uint64_t splitmix64(uint64_t& s) {
-  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-
+// Sequential keystream over chacha20_block (secret keys and encryption); the evaluation
+// keys use the counter-based samplers of hb_sample.hpp.
 ChaCha20::ChaCha20(uint64_t seed, uint32_t kind, uint32_t index) {
-  uint64_t s = seed;
-  for (int i = 0; i < 4; i++) {
-    uint64_t w = splitmix64(s);
-    key_[2 * i] = (uint32_t)w;
-    key_[2 * i + 1] = (uint32_t)(w >> 32);
-  }
+  derive_key(seed, key_);
   nonce_[0] = kind;
   nonce_[1] = index;
   nonce_[2] = 0x68623031u;  // domain tag
 }
 
 void ChaCha20::refill() {
-  uint32_t x[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key_[0], key_[1],
-                    key_[2],     key_[3],     key_[4],     key_[5],     key_[6], key_[7],
-                    counter_,    nonce_[0],   nonce_[1],   nonce_[2]};
-  uint32_t in[16];
-  std::copy(x, x + 16, in);
-  auto qr = [&](int a, int b, int c, int d) {
-    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 16);
-    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 12);
-    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 8);
-    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 7);
-  };
-  for (int i = 0; i < 10; i++) {
-    qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
-    qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
-  }
-  for (int i = 0; i < 16; i++) buf_[i] = x[i] + in[i];
+  chacha20_block(key_, counter_, nonce_[0], nonce_[1], nonce_[2], buf_);
   counter_++;
   pos_ = 0;
 }
@@ -258,12 +233,12 @@ uint64_t ksk_words(const ParamSet& p) { return (uint64_t)p.ks_len * 2 * p.N; }
 
 // NTTRLWE::generate_random_{zero,value}_sample (lattice/src/rlwe.rs:1215-1252):
 // a uniform in the NTT domain, e Gaussian in the coefficient domain (+value at X^0),
-// b = NTT(e) + a ⊙ s.
-static void rlwe_sample(ChaCha20& r, const HostNtt& ntt, const uint64_t* s_ntt, double sigma,
-                        uint64_t value, uint64_t* a, uint64_t* b) {
+// b = NTT(e) + a ⊙ s. Coefficient k draws from block `first + k` of stream (kind, row).
+static void rlwe_sample(const uint32_t key[8], uint32_t kind, uint32_t row, uint32_t first,
+                        const HostNtt& ntt, const uint64_t* s_ntt, double sigma, uint64_t value,
+                        uint64_t* a, uint64_t* b) {
   const ParamSet& p = ntt.p;
-  for (uint32_t k = 0; k < ntt.N; k++) a[k] = sample_uniform(r, p);
-  for (uint32_t k = 0; k < ntt.N; k++) b[k] = sample_gaussian(r, sigma, p.q);
+  for (uint32_t k = 0; k < ntt.N; k++) key_coefficient(key, kind, row, first + k, p.q, sigma, a + k, b + k);
   b[0] = ntt.add(b[0], value);
   ntt.fwd(b);
   for (uint32_t k = 0; k < ntt.N; k++) b[k] = ntt.add(b[k], ntt.mul(a[k], s_ntt[k]));
@@ -272,15 +247,17 @@ static void rlwe_sample(ChaCha20& r, const HostNtt& ntt, const uint64_t* s_ntt, 
 // BinaryBlindRotationKey::generate (fhe_core/src/blind_rotation/rlwe/binary.rs:207-238):
 // RGSW(0) or RGSW(1) per LWE key bit, via NTTRGSW::generate_random_{zero,one}_sample
 // (lattice/src/rgsw.rs:343-392) and NTTGadgetRLWE generators (gadget/rlwe.rs:478-551).
+// Sample (c, j) of key bit i draws blocks [(c L + j) N, (c L + j + 1) N) of stream (BSK, i).
 void generate_bsk(const SecretKey& sk, uint64_t* bsk) {
   const ParamSet& p = sk.p;
   HostNtt ntt;
   ntt.init(p, ilog2(p.N));
   const uint32_t N = p.N, L = p.br_len;
+  uint32_t key[8];
+  derive_key(sk.seed, key);
   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   auto work = [&](unsigned tid) {
     for (uint32_t i = tid; i < p.n; i += nth) {
-      ChaCha20 r(sk.seed, STREAM_BSK, i);
       const bool one = sk.lwe_sk[i] != 0;
       for (uint32_t c = 0; c < 2; c++) {
         uint64_t basis_power = 1;
@@ -289,7 +266,8 @@ void generate_bsk(const SecretKey& sk, uint64_t* bsk) {
           uint64_t* b = a + N;
           // c = 1 (c_m): value sample of B^j when s_i = 1 (gadget/rlwe.rs:494-527).
           uint64_t value = (one && c == 1) ? basis_power : 0;
-          rlwe_sample(r, ntt, sk.ring_sk_ntt.data(), p.ring_sigma, value, a, b);
+          rlwe_sample(key, STREAM_BSK, i, (c * L + j) * N, ntt, sk.ring_sk_ntt.data(),
+                      p.ring_sigma, value, a, b);
           // c = 0 (c_neg_s_m): zero sample with a += B^j when s_i = 1 (gadget/rlwe.rs:530-551).
           if (one && c == 0)
             for (uint32_t k = 0; k < N; k++) a[k] = ntt.add(a[k], basis_power);
@@ -305,21 +283,23 @@ void generate_bsk(const SecretKey& sk, uint64_t* bsk) {
 
 // KeySwitchingRLWEKey::generate (fhe_core/src/key_switch/rlwe.rs:37-112), branch
 // n.next_power_of_two() == N: RLWE samples under NTT(s_lwe ‖ 0), b += B_ks^i · NTT(s_ring).
+// Sample i draws blocks [0, N) of stream (KSK, i).
 void generate_ksk(const SecretKey& sk, uint64_t* ksk) {
   const ParamSet& p = sk.p;
   HostNtt ntt;
   ntt.init(p, ilog2(p.N));
   const uint32_t N = p.N;
+  uint32_t key[8];
+  derive_key(sk.seed, key);
   std::vector<uint64_t> lwe_ntt(N, 0);
   for (uint32_t k = 0; k < p.n; k++) lwe_ntt[k] = sk.lwe_sk[k];
   ntt.fwd(lwe_ntt.data());
   std::vector<uint64_t> s = sk.ring_sk_ntt;
   const uint64_t basis = 1ull << p.ks_bits;
   for (uint32_t i = 0; i < p.ks_len; i++) {
-    ChaCha20 r(sk.seed, STREAM_KSK, i);
     uint64_t* a = ksk + (uint64_t)i * 2 * N;
     uint64_t* b = a + N;
-    rlwe_sample(r, ntt, lwe_ntt.data(), p.ks_sigma, 0, a, b);
+    rlwe_sample(key, STREAM_KSK, i, 0, ntt, lwe_ntt.data(), p.ks_sigma, 0, a, b);
     for (uint32_t k = 0; k < N; k++) b[k] = ntt.add(b[k], s[k]);
     if (i + 1 < p.ks_len)
       for (uint32_t k = 0; k < N; k++) s[k] = ntt.mul(s[k], basis);

@@ -127,6 +127,70 @@ class SecretKeyPack:
         check(lib().hb_keygen_host(self._h, ptr(bsk), ptr(ksk)))
         return bsk, ksk
 
+    def generate_keys_device(self) -> tuple[np.ndarray, np.ndarray]:
+        """The same keys generated on the GPU (hb_keygen_dev), copied to the host."""
+        bw, kw = C.c_uint64(), C.c_uint64()
+        check(lib().hb_key_words(self.params.set_id, C.byref(bw), C.byref(kw)))
+        L = lib()
+        bufs = [C.c_void_p(), C.c_void_p()]
+        try:
+            for b, w in zip(bufs, (bw.value, kw.value)):
+                check(L.hb_dev_alloc(w * 8, C.byref(b)))
+            check(L.hb_keygen_dev(self._h, bufs[0], bufs[1], None))
+            out = []
+            for b, w in zip(bufs, (bw.value, kw.value)):
+                a = np.zeros(w, dtype=np.uint64)
+                check(L.hb_memcpy_d2h(a.ctypes.data, b, a.nbytes))
+                out.append(a)
+        finally:
+            for b in bufs:
+                if b.value:
+                    L.hb_dev_free(b)
+        return out[0], out[1]
+
+
+def _serialized(fn, *args) -> bytes:
+    size = C.c_size_t()
+    check(fn(*args, None, 0, C.byref(size)))
+    buf = C.create_string_buffer(size.value)
+    check(fn(*args, buf, size.value, C.byref(size)))
+    return buf.raw[:size.value]
+
+
+def serialize_keys(set_id: int, bsk, ksk) -> bytes:
+    """Canonical BSK + KSK in the versioned format (hb_keys_serialize)."""
+    bsk = np.ascontiguousarray(bsk, np.uint64)
+    ksk = np.ascontiguousarray(ksk, np.uint64)
+    return _serialized(lib().hb_keys_serialize, set_id, ptr(bsk), ptr(ksk))
+
+
+def deserialize_keys(data: bytes) -> tuple[int, np.ndarray, np.ndarray]:
+    """(set id, bsk, ksk) from hb_keys_serialize's format, checked."""
+    set_id = C.c_int()
+    check(lib().hb_keys_deserialize(data, len(data), C.byref(set_id), None, None))
+    bw, kw = C.c_uint64(), C.c_uint64()
+    check(lib().hb_key_words(set_id.value, C.byref(bw), C.byref(kw)))
+    bsk = np.zeros(bw.value, dtype=np.uint64)
+    ksk = np.zeros(kw.value, dtype=np.uint64)
+    check(lib().hb_keys_deserialize(data, len(data), None, ptr(bsk), ptr(ksk)))
+    return set_id.value, bsk, ksk
+
+
+def serialize_ciphertexts(set_id: int, cts) -> bytes:
+    """LWE ciphertexts (count, n+1) in the versioned format (hb_lwe_serialize)."""
+    cts = np.ascontiguousarray(np.atleast_2d(cts), np.uint64)
+    return _serialized(lib().hb_lwe_serialize, set_id, cts.shape[0] if cts.size else 0, ptr(cts))
+
+
+def deserialize_ciphertexts(data: bytes) -> tuple[int, np.ndarray]:
+    """(set id, ciphertexts of shape (count, n+1)) from hb_lwe_serialize's format."""
+    set_id, count = C.c_int(), C.c_size_t()
+    check(lib().hb_lwe_deserialize(data, len(data), C.byref(set_id), C.byref(count), None, 0))
+    P = Parameters.of(set_id.value)
+    out = np.zeros((count.value, P.n + 1), dtype=np.uint64)
+    check(lib().hb_lwe_deserialize(data, len(data), None, None, ptr(out), out.size))
+    return set_id.value, out
+
 
 class KeyGen:
     @staticmethod
@@ -199,6 +263,22 @@ class Evaluator:
 
     def parameters(self) -> Parameters:
         return self.params
+
+    def serialize(self) -> bytes:
+        """The evaluation key in the versioned format (hb_ek_serialize)."""
+        return _serialized(lib().hb_ek_serialize, self._h)
+
+    @classmethod
+    def deserialize(cls, data: bytes, device: int = 0) -> "Evaluator":
+        """An evaluator over keys read from hb_ek_serialize's format (hb_ek_deserialize)."""
+        ev = cls.__new__(cls)
+        h = C.c_void_p()
+        check(lib().hb_ek_deserialize(data, len(data), device, C.byref(h)))
+        ev._h = h
+        p = HbParams()
+        check(lib().hb_ek_params(h, C.byref(p)))
+        ev.params = Parameters.of(p.set)
+        return ev
 
     def export_keys(self) -> tuple[np.ndarray, np.ndarray]:
         bw, kw = C.c_uint64(), C.c_uint64()

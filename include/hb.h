@@ -103,13 +103,38 @@ int hb_encrypt(hb_sk_t* sk, uint32_t bit, uint64_t* out_lwe);
 int hb_decrypt(const hb_sk_t* sk, const uint64_t* lwe, uint32_t* bit, uint64_t* noise);
 
 /* Evaluator::new (tfhe/src/bfhe/evaluate.rs:72-76) → EvaluationKey::new (:26-39):
- * generates BSK + KSK on the host and makes them resident in HBM of `device`. */
+ * generates BSK + KSK on the GPU (hb_keygen_dev) and keeps them resident in HBM of
+ * `device`, with a canonical host copy for hb_ek_export. */
 int hb_ek_create(const hb_sk_t* sk, int device, hb_ek_t** out);
 /* Same, from keys in the canonical layout (e.g. produced elsewhere). */
 int hb_ek_import(int set, const uint64_t* bsk, const uint64_t* ksk, int device, hb_ek_t** out);
 int hb_ek_export(const hb_ek_t* ek, uint64_t* bsk, uint64_t* ksk);
 /* Host-only key generation into caller buffers (canonical layout), no device needed. */
 int hb_keygen_host(const hb_sk_t* sk, uint64_t* bsk, uint64_t* ksk);
+/* The same keys generated on the GPU (BinaryBlindRotationKey::generate,
+ * fhe_core/src/blind_rotation/rlwe/binary.rs:207-238; KeySwitchingRLWEKey::generate,
+ * fhe_core/src/key_switch/rlwe.rs:37-112) into device arrays of hb_key_words words,
+ * canonical layout, word for word equal to hb_keygen_host's; returns when they are written.
+ * hb_ek_create generates its keys this way. */
+int hb_keygen_dev(const hb_sk_t* sk, uint64_t* bsk_dev, uint64_t* ksk_dev, void* stream);
+
+/* Versioned serialized forms (no reference counterpart): a 72-byte header (magic "HBTF",
+ * format version 1, kind, the parameter set and its parameters, count, payload length, a
+ * checksum) then canonical words; the layout is in hb_format.cpp. With buf == NULL the
+ * *_serialize calls only report the size. Reading refuses other versions, parameters that
+ * differ from the set's, a wrong length or checksum, and non-canonical words (HB_ERR_ARG). */
+int hb_keys_serialize(int set, const uint64_t* bsk, const uint64_t* ksk, void* buf, size_t cap,
+                      size_t* size);
+int hb_keys_deserialize(const void* buf, size_t size, int* set, uint64_t* bsk, uint64_t* ksk);
+int hb_ek_serialize(const hb_ek_t* ek, void* buf, size_t cap, size_t* size);
+/* hb_keys_deserialize + hb_ek_import. */
+int hb_ek_deserialize(const void* buf, size_t size, int device, hb_ek_t** out);
+/* `count` LWE ciphertexts of n+1 canonical words each. hb_lwe_deserialize with lwes == NULL
+ * reports set and count only; cap_words is the room in lwes. */
+int hb_lwe_serialize(int set, size_t count, const uint64_t* lwes, void* buf, size_t cap,
+                     size_t* size);
+int hb_lwe_deserialize(const void* buf, size_t size, int* set, size_t* count, uint64_t* lwes,
+                       size_t cap_words);
 int hb_ek_destroy(hb_ek_t* ek);
 int hb_ek_params(const hb_ek_t* ek, hb_params_t* out);
 
