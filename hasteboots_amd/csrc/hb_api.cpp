@@ -759,7 +759,7 @@ int hb_mixed_batch_dev(const hb_batch_part_t* parts, size_t nparts, void* stream
   HIP_TRY(hipStreamSynchronize(s));  // `args` is pageable host memory
   int rc = timer_start(s);
   if (rc) return rc;
-  HIP_TRY(launch_mixed_kernel(kind, E.mixed_parts, (int)args.size(), blocks, narrow, s));
+  HIP_TRY(launch_mixed_kernel(kind, E.mixed_parts, (int)args.size(), blocks, narrow, sets, s));
   rc = timer_stop(s);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(E.mixed_ev, s));

@@ -131,13 +131,22 @@ inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, b
   return gates <= E.simd_slots ? MixedKernel::one_wave : MixedKernel::batch;
 }
 inline uint32_t mixed_gpw(MixedKernel k) { return k == MixedKernel::batch ? kMixedBatchGpw : 1; }
+// The kernel class of a launch over the parameter sets in `sets` (launch_mixed_batch's cls):
+// one class when every part is Goldilocks with 8-bit digits, another Goldilocks set
+// (Zama), or BabyBear; 0 when they mix.
+inline int mixed_class(int sets) {
+  if (sets == (1 << SET_GOLDILOCKS)) return 1;
+  if (sets == (1 << SET_ZAMA)) return 2;
+  if (sets == (1 << SET_BABYBEAR)) return 3;
+  return 0;
+}
 inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, int nparts,
-                                      uint32_t blocks, bool narrow, hipStream_t s) {
+                                      uint32_t blocks, bool narrow, int sets, hipStream_t s) {
   switch (k) {
     case MixedKernel::split:
       return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), s);
     case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
-    default: return launch_mixed_batch(parts_dev, nparts, blocks, s);
+    default: return launch_mixed_batch(parts_dev, nparts, blocks, mixed_class(sets), s);
   }
 }
 

@@ -730,7 +730,10 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
 // per workgroup, sharing its twiddle tables, with registers allotted for WPS waves per
 // SIMD. Here first_block counts workgroups, so every part starts on a workgroup boundary;
 // the waves past a part's count leave after the twiddle barrier.
-template <int GPW, int WPS>
+// CLS restricts the parts to one class so the kernel holds one body and allocates registers
+// as the dedicated batched kernel does: 1 Goldilocks with 8-bit digits, 2 the other
+// Goldilocks sets, 3 BabyBear; 0 takes any mix (runtime branch).
+template <int GPW, int WPS, int CLS>
 __global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
 void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[GPW][1024 + 64];
@@ -742,14 +745,18 @@ void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
   const BootArgs A = parts[p];
   const int w = threadIdx.x >> 6;
   const u32 g = (b - A.first_block) * GPW + w;
-  if (A.is_goldilocks) {
+  if (CLS == 1 || CLS == 2 || (CLS == 0 && A.is_goldilocks)) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * GPW);
     if (g >= A.count) return;
-    if (dig8_set(A))
+    if constexpr (CLS == 1)
+      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+    else if constexpr (CLS == 2)
+      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+    else if (dig8_set(A))
       bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
     else
       bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
-  } else {
+  } else if constexpr (CLS == 0 || CLS == 3) {
     u32* t = reinterpret_cast<u32*>(&twl[0][0]);
     stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * GPW);
     if (g >= A.count) return;
@@ -1015,10 +1022,26 @@ hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, 
   return hipGetLastError();
 }
 
-hipError_t launch_mixed_batch(const BootArgs* parts_dev, int nparts, u32 total_blocks,
+hipError_t launch_mixed_batch(const BootArgs* parts_dev, int nparts, u32 total_blocks, int cls,
                               hipStream_t s) {
-  hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2>), dim3(total_blocks),
-                     dim3(64 * kMixedBatchGpw), 0, s, parts_dev, nparts);
+  const dim3 grid(total_blocks), block(64 * kMixedBatchGpw);
+  switch (cls) {
+    case 1:
+      hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2, 1>), grid, block, 0, s, parts_dev,
+                         nparts);
+      break;
+    case 2:
+      hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2, 2>), grid, block, 0, s, parts_dev,
+                         nparts);
+      break;
+    case 3:
+      hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2, 3>), grid, block, 0, s, parts_dev,
+                         nparts);
+      break;
+    default:
+      hipLaunchKernelGGL((mixed_batch_kernel<kMixedBatchGpw, 2, 0>), grid, block, 0, s, parts_dev,
+                         nparts);
+  }
   return hipGetLastError();
 }
 

@@ -63,9 +63,11 @@ hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_ga
 // Throughput-mode mixed launch: kMixedBatchGpw gates of one part per workgroup, two waves
 // per SIMD (the batched kernel's (4, 2) shape). first_block counts workgroups here, so
 // part k occupies ceil(count_k / kMixedBatchGpw) workgroups from its first_block.
+// cls: 1 every part Goldilocks with 8-bit digits, 2 every part another Goldilocks set,
+// 3 every part BabyBear (one kernel body each), 0 any mix.
 constexpr uint32_t kMixedBatchGpw = 4;
 hipError_t launch_mixed_batch(const BootArgs* parts_dev, int nparts, uint32_t total_blocks,
-                              hipStream_t s);
+                              int cls, hipStream_t s);
 // Sets first_block of parts[0..n) for a launch with `gpw` gates per workgroup (1 for
 // launch_mixed and launch_mixed_split) and returns the workgroup count.
 inline uint32_t assign_blocks(BootArgs* parts, size_t n, uint32_t gpw) {

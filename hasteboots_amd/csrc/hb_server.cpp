@@ -279,7 +279,7 @@ int launch(hb_server* S, Slot& sl) {
   HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
-  HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sl.s));
+  HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sets, sl.s));
   HIP_TRY(hipEventRecord(sl.ev1, sl.s));
   sl.busy = true;
   sl.gates = used;

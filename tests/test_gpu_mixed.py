@@ -137,3 +137,26 @@ def test_mixed_throughput_shape_every_output(hb, orc, keyset):
         ref = orc.OracleKeys(s, bsk, ksk).gate_batch(0, c0, c1, threads=min(16, os.cpu_count() or 1))
         bad = np.flatnonzero(~np.all(out == ref, axis=1))
         assert bad.size == 0, (s, bad[:8])
+
+
+@pytest.mark.parametrize("set_id", [0, 1, 2], ids=["babybear", "goldilocks", "zama"])
+def test_single_set_throughput_shape_matches_batched_kernel(hb, orc, keyset, set_id):
+    """A mixed launch of one parameter set above one gate per SIMD runs that set's own
+    kernel body (one class); two clients' parts of 550 gates each: every output equals the
+    dedicated batched launch's, and a sample the oracle's."""
+    rng = stream(44 + set_id, "single-class")
+    parts, data = [], []
+    for seed in (0, 1):
+        P, sk, bsk, ksk = keyset(set_id, seed)
+        ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+        enc = hb.Encryptor(sk)
+        b0, b1 = rng.integers(0, 2, size=(2, 550))
+        c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
+        parts.append((ev, "nand", c0, c1))
+        data.append((ev, bsk, ksk, c0, c1))
+    outs = hb.mixed_batch(parts)
+    for (ev, bsk, ksk, c0, c1), out in zip(data, outs):
+        assert np.array_equal(out, ev.gate_batch("nand", c0, c1))
+        pick = np.unique(rng.integers(0, 550, size=8))
+        ref = orc.OracleKeys(set_id, bsk, ksk).gate_batch(0, c0[pick], c1[pick], threads=8)
+        assert np.array_equal(out[pick], ref)
