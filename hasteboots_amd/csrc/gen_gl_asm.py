@@ -197,6 +197,27 @@ def ct_p2_chain(tag: str, base: int, typ: str) -> Chain:
     return c
 
 
+def ct_triv_chain(tag: str, base: int, typ: str) -> Chain:
+    """Twiddle 1: x' = x + y, y' = x - y. TC: y canonical; TN: y in [0, 2^64), first made
+    canonical (y + EPS carries exactly when y >= P, and then is y - P)."""
+    c = Chain(tag, base)
+    x0, x1, y0, y1 = f"%x{tag}0", f"%x{tag}1", f"%y{tag}0", f"%y{tag}1"
+    if typ == "TC":
+        ct_addsub(c, tag, x0, x1, y0, y1)
+        return c
+    c.add(f"v_add_co_u32 {{A.lo}}, {{S1}}, {y0}, -1", ["A.lo", "S1"], [y0])
+    c.add(f"v_addc_co_u32 {{A.hi}}, {{S1}}, {y1}, 0, {{S1}}", ["A.hi", "S1"], [y1, "S1"])
+    c.add(f"v_cndmask_b32 {{A.lo}}, {y0}, {{A.lo}}, {{S1}}", ["A.lo"], [y0, "A.lo", "S1"])
+    c.add(f"v_cndmask_b32 {{A.hi}}, {y1}, {{A.hi}}, {{S1}}", ["A.hi"], [y1, "A.hi", "S1"])
+    ct_addsub(c, tag, x0, x1)
+    return c
+
+
+def ct_any_chain(tag: str, base: int, typ: str) -> Chain:
+    """CT butterfly whose twiddle is +-2^e (A, BL, BH) or 1 (TC, TN)."""
+    return ct_triv_chain(tag, base, typ) if typ in ("TC", "TN") else ct_p2_chain(tag, base, typ)
+
+
 def gs_p2_chain(tag: str, base: int, typ: str) -> Chain:
     """x, y canonical: x' = canon(x + y) (two halves out), y' = (x - y) * 2^e."""
     c = Chain(tag, base)
@@ -217,12 +238,14 @@ def ct_chain(tag: str, base: int, wkind: str) -> Chain:
     return c
 
 
-def ct_addsub(c: Chain, tag: str, x0: str, x1: str):
-    """Outputs %X = x + A, %Y = x - A for canonical A, one EPS correction each."""
-    c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {{A.lo}}", ["U.lo", "S1"], [x0, "A.lo"])
-    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {{A.lo}}", ["V.lo", "S2"], [x0, "A.lo"])
-    c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {{A.hi}}, {{S1}}", ["U.hi", "S1"], [x1, "A.hi", "S1"])
-    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {{A.hi}}, {{S2}}", ["V.hi", "S2"], [x1, "A.hi", "S2"])
+def ct_addsub(c: Chain, tag: str, x0: str, x1: str, a0: str = "A.lo", a1: str = "A.hi"):
+    """Outputs %X = x + a, %Y = x - a for canonical a (default the pair A), one EPS
+    correction each."""
+    w0, w1 = word(a0), word(a1)
+    c.add(f"v_add_co_u32 {{U.lo}}, {{S1}}, {x0}, {w0}", ["U.lo", "S1"], [x0, a0])
+    c.add(f"v_sub_co_u32 {{V.lo}}, {{S2}}, {x0}, {w0}", ["V.lo", "S2"], [x0, a0])
+    c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {x1}, {w1}, {{S1}}", ["U.hi", "S1"], [x1, a1, "S1"])
+    c.add(f"v_subb_co_u32 {{V.hi}}, {{S2}}, {x1}, {w1}, {{S2}}", ["V.hi", "S2"], [x1, a1, "S2"])
     c.add("v_cndmask_b32 {T1}, 0, -1, {S1}", ["T1"], ["S1"])
     c.add("v_cndmask_b32 {T2}, 0, -3, {S2}", ["T2"], ["S2"])
     c.add(f"v_mad_u64_u32 {{%X{tag}}}, {{D}}, {{T1}}, 1, {{U}}", [f"%X{tag}"], ["T1", "U"])
@@ -273,6 +296,13 @@ def mac_chain(tag: str, base: int) -> Chain:
     c.add(f"v_addc_co_u32 {{U.hi}}, {{S1}}, {p1}, {{A.hi}}, {{S1}}", ["U.hi", "S1"], [p1, "A.hi", "S1"])
     c.add("v_cndmask_b32 {T1}, 0, -1, {S1}", ["T1"], ["S1"])
     c.add(f"v_mad_u64_u32 {{%P{tag}}}, {{D}}, {{T1}}, 1, {{U}}", [f"%P{tag}"], ["T1", "U"])
+    return c
+
+
+def mul_chain(tag: str, base: int) -> Chain:
+    """r = montmul(x, w), canonical; x in [0, 2^64), w < P (Montgomery constant form)."""
+    c = Chain(tag, base)
+    montmul(c, f"%x{tag}0", f"%x{tag}1", f"%w{tag}0", f"%w{tag}1", f"%R{tag}")
     return c
 
 
@@ -538,6 +568,24 @@ def main():
             parts.append(emit(f"gsp2_{ta}_{tb}", [a, b], outputs, inputs,
                               f"Two GS butterflies with power-of-two twiddles ({ta}, {tb}) on "
                               "canonical inputs: (x, y) <- (x + y, (x - y) * 2^e)."))
+    # CT pairs with a twiddle of 1 in at least one chain (the twisted forward transform)
+    for ta in ("A", "BL", "BH", "TN", "TC"):
+        for tb in ("A", "BL", "BH", "TN", "TC"):
+            if ta not in ("TN", "TC") and tb not in ("TN", "TC"):
+                continue
+            inputs = []
+            for t, typ in (("a", ta), ("b", tb)):
+                inputs += [(f"x{t}0", "v", f"(u32)x{t}"), (f"x{t}1", "v", f"(u32)(x{t} >> 32)"),
+                           (f"y{t}0", "v", f"(u32)y{t}"), (f"y{t}1", "v", f"(u32)(y{t} >> 32)")]
+                if typ not in ("TN", "TC"):
+                    inputs += [(f"s{t}1", "n", f"S{t.upper()}1"), (f"s{t}2", "n", f"S{t.upper()}2")]
+            inputs.append(("k55", "s", "0x55555555u"))
+            a, b = ct_any_chain("a", TEMP_BASE, ta), ct_any_chain("b", TEMP_BASE + 12, tb)
+            outputs = [("Xa", "=v", "Xa"), ("Ya", "=v", "Ya"), ("Xb", "=v", "Xb"),
+                       ("Yb", "=v", "Yb")]
+            parts.append(emit(f"ctp2_{ta}_{tb}", [a, b], outputs, inputs,
+                              f"Two CT butterflies ({ta}, {tb}): (x, y) <- (x + t, x - t), t = y "
+                              "for TC (y canonical) and TN (y any word), else y * 2^e by shifts."))
     a, b = mac_chain("a", TEMP_BASE), mac_chain("b", TEMP_BASE + 12)
     outputs = [("Pa", "=v", "Pa"), ("Pb", "=v", "Pb")]
     inputs = [("x0", "v", "(u32)x"), ("x1", "v", "(u32)(x >> 32)")]
@@ -547,6 +595,15 @@ def main():
     inputs.append(("k55", "s", "0x55555555u"))
     parts.append(emit("mac2", [a, b], outputs, inputs,
                       "(pa, pb) += (x * ka, x * kb), keys in Montgomery constant form."))
+    a, b = mul_chain("a", TEMP_BASE), mul_chain("b", TEMP_BASE + 12)
+    outputs = [("Ra", "=v", "Ra"), ("Rb", "=v", "Rb")]
+    inputs = []
+    for t in "ab":
+        inputs += [(f"x{t}0", "v", f"(u32)x{t}"), (f"x{t}1", "v", f"(u32)(x{t} >> 32)"),
+                   (f"w{t}0", "v", f"(u32)w{t}"), (f"w{t}1", "v", f"(u32)(w{t} >> 32)")]
+    inputs.append(("k55", "s", "0x55555555u"))
+    parts.append(emit("mul2", [a, b], outputs, inputs,
+                      "(ra, rb) <- (xa * wa, xb * wb), constants in Montgomery form; canonical."))
     text = "\n".join(parts)
     if len(sys.argv) > 1 and sys.argv[1] == "--check":
         return 0 if open(OUT).read() == text else 1

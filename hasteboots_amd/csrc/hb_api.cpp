@@ -177,6 +177,34 @@ int hb::ensure_engine(int device) {
       E.tables.ninv[f][lg - 10] = f ? Gl::to_mont(ntt.n_inv) : Bb::to_mont((uint32_t)ntt.n_inv);
     }
   }
+  {  // twisted digit-transform table (ntt_fwd_digits): beta_g^p, then the pass-3 twiddles
+    ParamSet p;
+    param_set(SET_GOLDILOCKS, &p);
+    HostNtt ntt;
+    ntt.init(p, 10);
+    auto psi_pow = [&](uint64_t e) {
+      uint64_t r = 1, b = ntt.psi;
+      for (e %= 2048; e; e >>= 1, b = ntt.mul(b, b))
+        if (e & 1) r = ntt.mul(r, b);
+      return r;
+    };
+    auto rev = [](uint32_t x, int bits) {
+      uint32_t r = 0;
+      for (int i = 0; i < bits; i++) r |= ((x >> i) & 1u) << (bits - 1 - i);
+      return r;
+    };
+    std::vector<uint64_t> t(kTwxWords, 0);
+    for (uint32_t pos = 0; pos < 1024; pos++)
+      t[68 * (pos >> 6) + (pos & 63)] =
+          Gl::to_mont(psi_pow((uint64_t)rev(16 + (pos >> 6), 5) * (pos & 63)));
+    for (uint32_t ip = 0; ip < 16; ip++) {
+      t[1088 + ip] = Gl::to_mont(psi_pow(rev(256 + ip, 10) + 2048 - 2));
+      for (uint32_t h = 0; h < 2; h++)
+        t[1104 + 2 * ip + h] = Gl::to_mont(psi_pow(rev(512 + 2 * ip + h, 10) + 2048 - 1));
+    }
+    HIP_TRY(hipMalloc(&E.tables.twx, t.size() * 8));
+    HIP_TRY(hipMemcpy(E.tables.twx, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+  }
   E.tables.ready = true;
   E.device = device;
   {
@@ -357,7 +385,7 @@ BootArgs hb::make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* 
   a.bsk = ek->bsk_dev;
   a.ksk = ek->ksk_dev;
   const int f = p.is_goldilocks ? 1 : 0;
-  a.tw_fwd = E.tables.tw[f][0][0];
+  a.tw_fwd = f ? E.tables.twx : E.tables.tw[f][0][0];  // the digit transforms' table
   a.tw_inv = E.tables.tw[f][0][1];
   a.c0 = c0;
   a.c1 = c1;
@@ -994,7 +1022,8 @@ int hb_external_product_dev(int field, uint32_t log_n, uint32_t bits, uint32_t l
   }
   rc = timer_start(s);
   if (rc) return rc;
-  HIP_TRY(launch_extprod(field, rlwe_dev, count, conv, len, bits, E.tables.tw[field][0][0],
+  HIP_TRY(launch_extprod(field, rlwe_dev, count, conv, len, bits,
+                         field ? E.tables.twx : E.tables.tw[field][0][0],
                          E.tables.tw[field][0][1], s));
   rc = timer_stop(s);
   if (rc) return rc;

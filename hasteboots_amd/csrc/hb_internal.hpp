@@ -28,6 +28,9 @@ struct DeviceTables {
   bool ready = false;
   void* tw[2][2][2] = {};    // [field][logN-10][inverse]
   uint64_t ninv[2][2] = {};  // device-form N^-1 for the microbench inverse
+  // Goldilocks, N = 1024: the forward table of the twisted digit transform (kTwxWords
+  // words, ntt_fwd_digits), the kernels' BootArgs::tw_fwd for the Goldilocks sets
+  void* twx = nullptr;
 };
 
 // The process-wide engine: one device, its default stream, twiddle tables and the

@@ -135,10 +135,21 @@ __device__ __forceinline__ void mac_row(A (&pa)[E], A (&pb)[E],
   }
 }
 
+// Forward transform of gadget digits (every word below 2^16): the twisted transform for
+// Goldilocks (twv: the kTwxWords table of ntt_fwd_digits), the plain one for BabyBear (twv:
+// psi_rev). The slot values are the same either way.
+template <class F, int E, bool PF>
+__device__ __forceinline__ void digit_ntt(typename DF<F>::W (&x)[E], typename DF<F>::W* lds,
+                                          const typename DF<F>::W* twv, int lane) {
+  if constexpr (F::IS_GOLDILOCKS && E == 16)
+    ntt_fwd_digits<PF>(x, lds, twv, lane);
+  else
+    ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);
+}
+
 // Σ over one gadget of digit NTTs: for j < len, x = NTT(digit_j(src)), acc += x ⊙ key_j.
 // Decomposition: decompose_lsb_bits_inplace (native_polynomial.rs:340-348) on canonical
-// values, digit_j = (v >> j*bits) & mask. twv: the forward twiddles for per-lane reads (LDS
-// copy). S: the source word type (a field word, or u32 for a window of at most 32 bits of
+// values, digit_j = (v >> j*bits) & mask. twv: digit_ntt's forward table (LDS copy). S: the source word type (a field word, or u32 for a window of at most 32 bits of
 // digits). A: the accumulator type (field words for Goldilocks, lazy 64-bit sums for
 // BabyBear, folded after every second row: the transform's words are below 2P, so
 // 2 (2P-1)(P-1) + 2^60 < 2^64).
@@ -160,7 +171,7 @@ __device__ __forceinline__ void gadget_rows(A (&pa)[E], A (&pb)[E], const S (&sr
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (W)(src[e] >> sh) & mask;
-    ntt_fwd_wave<F, E, true, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
+    digit_ntt<F, E, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
     mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
     if constexpr (!F::IS_GOLDILOCKS) {
       if ((j & 1) == 1 && j + 1 < len) {
@@ -241,7 +252,7 @@ __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
     u64 x[E];
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (w[e >> 2] >> (8 * (e & 3))) & 0xFFu;
-    ntt_fwd_wave<Gl, E, true, PF>(x, lds, twv, lane);
+    ntt_fwd_digits<PF>(x, lds, twv, lane);
     mac_row<Gl, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
     d = dn;
   }
@@ -654,8 +665,9 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   if (wave == 0) keyswitch_extract<F, PF>(A, g, accA, accB, lds, twl, itwl, lane);
 }
 
-// Copies the forward and inverse twiddle tables (N words each) into LDS for the
-// per-lane reads of NTT passes 2 and 3.
+// Copies the forward and inverse twiddle tables (N words each; the Goldilocks forward table
+// of the digit transforms, ntt_fwd_digits, has kTwxWords) into LDS for the per-lane reads of
+// NTT passes 2 and 3. twl holds kTwxWords words.
 template <class W>
 __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, W* twl, W* itwl,
                                                int n, int tid, int nthreads) {
@@ -665,6 +677,9 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
     twl[k] = a[k];
     itwl[k] = b[k];
   }
+  // the Goldilocks forward table of the digit transforms has kTwxWords words
+  if (sizeof(W) == 8)
+    for (int k = n + tid; k < kTwxWords; k += nthreads) twl[k] = a[k];
   __syncthreads();
 }
 
@@ -677,15 +692,15 @@ void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[GPW][1024 + 64];
   __shared__ unsigned short ms[GPW][1024];
-  __shared__ W twl[2][1024];
-  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * GPW);
+  __shared__ W twl[kTwxWords];
+  __shared__ W itwl[1024];
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * GPW);
   const int w = threadIdx.x >> 6;
   const u32 g = blockIdx.x * GPW + w;
   if (g >= A.count) return;  // after the only workgroup barrier
   // Twiddles a stage ahead with one wave per SIMD, and for BabyBear, whose 32-bit words
   // leave the registers for it (4096 gates at (4,2): 55.5 -> 54.9 ms).
-  bootstrap_gate<F, WPS == 1 || !F::IS_GOLDILOCKS, DIG8>(A, (int)g, lds[w], ms[w], twl[0],
-                                                        twl[1]);
+  bootstrap_gate<F, WPS == 1 || !F::IS_GOLDILOCKS, DIG8>(A, (int)g, lds[w], ms[w], twl, itwl);
 }
 
 // Latency-mode launch: one gate per workgroup of WAVES wavefronts (bootstrap_gate_split).
@@ -694,11 +709,12 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   using W = typename DF<F>::W;
   __shared__ W lds[WAVES][1024 + 64];
   __shared__ unsigned short ms[1024];
-  __shared__ W twl[2][1024];
+  __shared__ W twl[kTwxWords];
+  __shared__ W itwl[1024];
   __shared__ W part[4 * 2 * 1024];
-  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
   bootstrap_gate_split<F, WAVES>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms, part,
-                                 twl[0], twl[1]);
+                                 twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -707,22 +723,24 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
 __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[1024 + 64];
   __shared__ unsigned short ms[1024];
-  __shared__ u64 twl[2][1024];
+  __shared__ u64 twl[kTwxWords];
+  __shared__ u64 itwl[1024];
   const u32 b = blockIdx.x;
   int p = 0;
   while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
   const BootArgs A = parts[p];
   const int g = (int)(b - A.first_block);
   if (A.is_goldilocks) {
-    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64);
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64);
     if (dig8_set(A))
-      bootstrap_gate<Gl, true, true>(A, g, lds, ms, twl[0], twl[1]);
+      bootstrap_gate<Gl, true, true>(A, g, lds, ms, twl, itwl);
     else
-      bootstrap_gate<Gl, true, false>(A, g, lds, ms, twl[0], twl[1]);
+      bootstrap_gate<Gl, true, false>(A, g, lds, ms, twl, itwl);
   } else {
-    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
-    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64);
-    bootstrap_gate<Bb>(A, g, reinterpret_cast<u32*>(lds), ms, t, t + 1024);
+    u32* t = reinterpret_cast<u32*>(twl);
+    u32* it = reinterpret_cast<u32*>(itwl);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64);
+    bootstrap_gate<Bb>(A, g, reinterpret_cast<u32*>(lds), ms, t, it);
   }
 }
 
@@ -738,7 +756,8 @@ __global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, W
 void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[GPW][1024 + 64];
   __shared__ unsigned short ms[GPW][1024];
-  __shared__ u64 twl[2][1024];
+  __shared__ u64 twl[kTwxWords];
+  __shared__ u64 itwl[1024];
   const u32 b = blockIdx.x;
   int p = 0;
   while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
@@ -746,21 +765,22 @@ void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
   const int w = threadIdx.x >> 6;
   const u32 g = (b - A.first_block) * GPW + w;
   if (CLS == 1 || CLS == 2 || (CLS == 0 && A.is_goldilocks)) {
-    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * GPW);
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * GPW);
     if (g >= A.count) return;
     if constexpr (CLS == 1)
-      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl, itwl);
     else if constexpr (CLS == 2)
-      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl, itwl);
     else if (dig8_set(A))
-      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl, itwl);
     else
-      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl[0], twl[1]);
+      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl, itwl);
   } else if constexpr (CLS == 0 || CLS == 3) {
-    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
-    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * GPW);
+    u32* t = reinterpret_cast<u32*>(twl);
+    u32* it = reinterpret_cast<u32*>(itwl);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64 * GPW);
     if (g >= A.count) return;
-    bootstrap_gate<Bb>(A, (int)g, reinterpret_cast<u32*>(lds[w]), ms[w], t, t + 1024);
+    bootstrap_gate<Bb>(A, (int)g, reinterpret_cast<u32*>(lds[w]), ms[w], t, it);
   }
 }
 
@@ -771,7 +791,8 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
     const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[WAVES][1024 + 64];
   __shared__ unsigned short ms[1024];
-  __shared__ u64 twl[2][1024];
+  __shared__ u64 twl[kTwxWords];
+  __shared__ u64 itwl[1024];
   __shared__ u64 part[4 * 2 * 1024];
   const u32 b = blockIdx.x;
   int p = 0;
@@ -780,13 +801,14 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   const int g = (int)(b - A.first_block);
   const int w = threadIdx.x >> 6;
   if (A.is_goldilocks) {
-    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl[0], twl[1], 1024, threadIdx.x, 64 * WAVES);
-    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl[0], twl[1]);
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
+    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
   } else {
-    u32* t = reinterpret_cast<u32*>(&twl[0][0]);
-    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, t + 1024, 1024, threadIdx.x, 64 * WAVES);
+    u32* t = reinterpret_cast<u32*>(twl);
+    u32* it = reinterpret_cast<u32*>(itwl);
+    stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64 * WAVES);
     bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64), ms,
-                                    reinterpret_cast<u32*>(part), t, t + 1024);
+                                    reinterpret_cast<u32*>(part), t, it);
   }
 }
 
@@ -880,9 +902,10 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
   using W = typename DF<F>::W;
   constexpr int E = 16, N = 1024;
   __shared__ W lds_all[4][N + N / 16];
-  __shared__ W twl[2][N];
+  __shared__ W twl[kTwxWords];
+  __shared__ W itwl[N];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  stage_twiddles<W>(tw_, itw_, twl[0], twl[1], N, threadIdx.x, 256);
+  stage_twiddles<W>(tw_, itw_, twl, itwl, N, threadIdx.x, 256);
   const size_t idx = (size_t)blockIdx.x * 4 + wave;
   if (idx >= count) return;
   const W* rgsw = static_cast<const W*>(rgsw_);
@@ -896,10 +919,10 @@ __global__ __launch_bounds__(256) void extprod_kernel(u64* __restrict__ rlwe, si
     W src[E];
 #pragma unroll
     for (int e = 0; e < E; e++) src[e] = (W)d[comp * N + lane + 64 * e];
-    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, twl[0], lane);
+    gadget_mac<F, E>(pa, pb, src, len, bits, rgsw + (size_t)comp * len * 2 * N, lds, twl, lane);
   }
-  ntt_inv_wave<F, E>(pa, lds, twl[1], lane);
-  ntt_inv_wave<F, E>(pb, lds, twl[1], lane);
+  ntt_inv_wave<F, E>(pa, lds, itwl, lane);
+  ntt_inv_wave<F, E>(pb, lds, itwl, lane);
 #pragma unroll
   for (int e = 0; e < E; e++) {
     d[lane + 64 * e] = (u64)pa[e];

@@ -19,10 +19,15 @@ enum GateKind : int {
   GATE_MUX = 8         // host-side sequence: NOT, two ANDs, OR-LUT bootstrap of the sum
 };
 
+// Words of the Goldilocks forward table of the digit transforms (the twisted transform of
+// hb_ntt.hpp: 16 blocks of 64 twist factors at a 68-word stride, 16 + 32 pass-3 twiddles);
+// BootArgs::tw_fwd of the Goldilocks sets.
+constexpr int kTwxWords = 16 * 68 + 48;
+
 struct BootArgs {
   const void* bsk;     // device layout, see hb_kernels.hip
   const void* ksk;
-  const void* tw_fwd;  // psi^bitrev(k), N words
+  const void* tw_fwd;  // psi^bitrev(k), N words (BabyBear); Goldilocks: kTwxWords words
   const void* tw_inv;  // psi^-bitrev(k), N words
   const uint64_t* c0;  // count*(n+1) canonical words
   const uint64_t* c1;  // may be null for GATE_BOOTSTRAP

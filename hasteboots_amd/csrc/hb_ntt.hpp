@@ -195,6 +195,13 @@ __device__ __forceinline__ void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb)
   pa = Pa, pb = Pb;
 }
 
+// (ra, rb) = (xa * wa, xb * wb) with Montgomery-form constants wa, wb (per lane); canonical.
+__device__ __forceinline__ void gl_mul2(u64& ra, u64& rb, u64 xa, u64 wa, u64 xb, u64 wb) {
+  u64 Ra, Rb, dummy, s1a, s2a, s1b, s2b;
+  HB_GL_MUL2_ASM;
+  ra = Ra, rb = Rb;
+}
+
 // Butterfly pairs whose twiddles are +-2^e (hb_gl_asm.inc power-of-two macros). TA/TB:
 // multiplier type 0 (A), 1 (BL), 2 (BH) with shift immediates S*1, S*2; NA/NB: the
 // twiddle is the negation of what the macro multiplies by. For CT a negated twiddle
@@ -220,6 +227,32 @@ __device__ __forceinline__ void gl_ct2_pow2(u64& xa_, u64& ya_, u64& xb_, u64& y
   xb_ = NB ? Yb : Xb, yb_ = NB ? Xb : Yb;
 }
 
+// CT pairs whose twiddles may also be 1: types 0-2 as above, 3 (TN: y any word) and 4 (TC: y
+// canonical) for a unit twiddle; N*: the twiddle is the negation (outputs swapped).
+#define HB_GL_CTX_CASE(A, B, NA_, NB_) \
+  else if constexpr (TA == NA_ && TB == NB_) { HB_GL_CTP2_##A##_##B##_ASM; }
+template <int TA, int SA1, int SA2, bool NA, int TB, int SB1, int SB2, bool NB>
+__device__ __forceinline__ void gl_ct2_unit(u64& xa_, u64& ya_, u64& xb_, u64& yb_) {
+  if constexpr (TA < 3 && TB < 3) {
+    gl_ct2_pow2<TA, SA1, SA2, NA, TB, SB1, SB2, NB>(xa_, ya_, xb_, yb_);
+  } else {
+    const u64 xa = xa_, ya = ya_, xb = xb_, yb = yb_;
+    u64 Xa, Ya, Xb, Yb, dummy, s1a, s2a, s1b, s2b;
+    if constexpr (false) {}
+    HB_GL_CTX_CASE(A, TN, 0, 3) HB_GL_CTX_CASE(A, TC, 0, 4)
+    HB_GL_CTX_CASE(BL, TN, 1, 3) HB_GL_CTX_CASE(BL, TC, 1, 4)
+    HB_GL_CTX_CASE(BH, TN, 2, 3) HB_GL_CTX_CASE(BH, TC, 2, 4)
+    HB_GL_CTX_CASE(TN, A, 3, 0) HB_GL_CTX_CASE(TN, BL, 3, 1) HB_GL_CTX_CASE(TN, BH, 3, 2)
+    HB_GL_CTX_CASE(TN, TN, 3, 3) HB_GL_CTX_CASE(TN, TC, 3, 4)
+    HB_GL_CTX_CASE(TC, A, 4, 0) HB_GL_CTX_CASE(TC, BL, 4, 1) HB_GL_CTX_CASE(TC, BH, 4, 2)
+    HB_GL_CTX_CASE(TC, TN, 4, 3) HB_GL_CTX_CASE(TC, TC, 4, 4)
+    else static_assert(TA < 0, "unknown twiddle type pair");
+    xa_ = NA ? Ya : Xa, ya_ = NA ? Xa : Ya;
+    xb_ = NB ? Yb : Xb, yb_ = NB ? Xb : Yb;
+  }
+}
+#undef HB_GL_CTX_CASE
+
 template <int TA, int SA1, int SA2, bool NA, int TB, int SB1, int SB2, bool NB>
 __device__ __forceinline__ void gl_gs2_pow2(u64& xa_, u64& ya_, u64& xb_, u64& yb_) {
   const u64 xa = NA ? ya_ : xa_, ya = NA ? xa_ : ya_;
@@ -235,6 +268,10 @@ __device__ __forceinline__ void gl_gs2_pow2(u64& xa_, u64& ya_, u64& xb_, u64& y
 __device__ inline void gl_mac2(u64& pa, u64& pb, u64 x, u64 ka, u64 kb) {
   pa = Gl::add(pa, Gl::mont_mul(x, ka));
   pb = Gl::add(pb, Gl::mont_mul(x, kb));
+}
+__device__ inline void gl_mul2(u64& ra, u64& rb, u64 xa, u64 wa, u64 xb, u64 wb) {
+  ra = Gl::mont_mul(xa, wa);
+  rb = Gl::mont_mul(xb, wb);
 }
 #endif
 
@@ -515,6 +552,131 @@ __device__ __forceinline__ void ntt_fwd_wave(typename DF<F>::W (&x)[E], typename
     if constexpr (!PF) load3(q);
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 2], t3[q][0], x[4 * q + 1], x[4 * q + 3], t3[q][0]);
     Bfly<F>::ct2_v(x[4 * q + 0], x[4 * q + 1], t3[q][1], x[4 * q + 2], x[4 * q + 3], t3[q][2]);
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Twisted forward transform of gadget digits (Goldilocks, N = 1024, every input word below
+// 2^16): the same slot values as ntt_fwd_wave<Gl, 16, true> with fewer general products.
+// Stages 0-3 (pass 1) split X^1024 + 1 into 16 blocks X^64 - Z_g, Z_g = psi^(2 bitrev10(16 +
+// g)), block g holding positions [64g, 64g + 64). Substituting X = beta_g Y with beta_g =
+// psi^bitrev5(16 + g) (so beta_g^32 = psi^bitrev10(16 + g)) turns block g into Y^64 - 1: its
+// words are multiplied by beta_g^p (the twist, p = position in the block), and the six
+// remaining Cooley-Tukey stages take the Y-domain twiddles psi^bitrev10(2^s + i) beta_g^-t
+// (t = 2^(9-s)), which do not depend on g and are 64th roots of unity, i.e. +-2^e or 1.
+// Pass 2 (stages 4-7) thus runs on shifts and additions at compile-time twiddles; pass 3
+// (stages 8, 9) keeps general products with 48 per-lane twiddles. The leaves are the same
+// evaluations in the same order. Table twx (Montgomery constant form, 1136 words): the
+// twist factor beta_g^p of block g, position p at 68 g + p (blocks padded like the exchange
+// buffer, so that a wave's reads spread over the LDS banks), [1088, 1104) the stage-8
+// twiddle of group ip, [1104, 1136) the stage-9 twiddles of groups (ip, h) at
+// 1104 + 2 ip + h, where ip = the group's index within its block (bits 5..2 of the
+// position). (kTwxWords, hb_kernels.hpp.)
+
+// Type of a Y-domain pass-2 twiddle for gl_ct2_unit: stage 4 + S, group IP within the block,
+// W = psi^(bitrev10(2^(4+S) + IP) - 2^(5-S)); CANON: the butterfly's y is canonical.
+struct UnitTw {
+  int type, s1, s2, neg;
+};
+template <int S, int IP, bool CANON>
+__host__ __device__ constexpr UnitTw twisted_pass2_twiddle() {
+  const u64 p = Gl::P;
+  const u64 psi = ct_powmod(Gl::GENERATOR, (p - 1) >> 11, p);
+  const u64 x = (ct_bitrev((1u << (4 + S)) + IP, 10) + 2048 - (32u >> S)) % 2048;
+  const u64 w = ct_powmod(psi, x, p);
+  u64 v = 1;
+  int e = -1;
+  for (int k = 0; k < 192; k++) {
+    if (v == w) {
+      e = k;
+      break;
+    }
+    v = ct_mulmod(v, 2, p);
+  }
+  int neg = 0;
+  if (e >= 96) neg = 1, e -= 96;
+  if (e == 0) return {CANON ? 4 : 3, 0, 0, neg};
+  if (e >= 1 && e < 32) return {0, e, 32 - e, neg};
+  if (e >= 32 && e < 96) {
+    const int j = e - 32;
+    if (j >= 1 && j < 32) return {1, j, 32 - j, !neg};
+    if (j >= 33 && j < 64) return {2, j - 32, 64 - j, !neg};
+  }
+  return {-1, 0, 0, 0};  // no shift form (2^32, 2^64): not a pass-2 twiddle
+}
+
+template <int S, int P>
+__device__ __forceinline__ void twisted_pass2_pair(u64 (&x)[16]) {
+  constexpr int half = 8 >> S;
+  constexpr int k0 = bf_lo(2 * P, half), k1 = bf_lo(2 * P + 1, half);
+  // after the twist every word is canonical: stage 4 takes y canonical
+  constexpr UnitTw a = twisted_pass2_twiddle<S, k0 / (2 * half), S == 0>();
+  constexpr UnitTw b = twisted_pass2_twiddle<S, k1 / (2 * half), S == 0>();
+  static_assert(a.type >= 0 && b.type >= 0, "pass-2 twiddle without a shift form");
+#if defined(__HIP_DEVICE_COMPILE__)
+  gl_ct2_unit<a.type, a.s1, a.s2, (bool)a.neg, b.type, b.s1, b.s2, (bool)b.neg>(
+      x[k0], x[k0 + half], x[k1], x[k1 + half]);
+#endif
+}
+
+template <int S, int... Ps>
+__device__ __forceinline__ void twisted_pass2_stage(u64 (&x)[16], std::integer_sequence<int, Ps...>) {
+  (twisted_pass2_pair<S, Ps>(x), ...);
+}
+
+// In: x[e] = digit word of position lane + 64e (< 2^16). Out: x[4q + r] = slot 4(lane + 64q) + r,
+// words in [0, 2^64). twx: the table above (normally an LDS copy). PF: issue the twist and
+// pass-3 table reads ahead of the exchanges (one wave per SIMD) instead of just before use.
+template <bool PF>
+__device__ __forceinline__ void ntt_fwd_digits(u64 (&x)[16], u64* lds, const u64* twx, int lane) {
+  constexpr int E = 16;
+  const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
+  // stage 0 exactly (inputs below 2^16; twiddle 2^48), then stages 1-3 at compile-time shifts
+#pragma unroll
+  for (int e = 0; e < E / 2; e++) {
+    const u32 a0 = (u32)x[e], t1 = (u32)x[e + E / 2] << 16;
+    x[e] = ((u64)t1 << 32) | a0;
+    x[e + E / 2] = ((u64)(~t1) << 32) | (a0 + 1u);
+  }
+  pass1_all<Gl, E, false, 1>(x, std::make_integer_sequence<int, 3>{});
+  // exchange to pass 2's layout: positions 64b + j0 + 4k, b = lane >> 2, j0 = lane & 3
+#pragma unroll
+  for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
+  const u64* tw1 = twx + 68 * (lane >> 2) + (lane & 3);
+  u64 w[E];
+  if constexpr (PF) {
+#pragma unroll
+    for (int k = 0; k < E; k++) w[k] = tw1[4 * k];
+  }
+  wave_sync();
+#pragma unroll
+  for (int k = 0; k < E; k++) x[k] = lds[b2 + 4 * k];
+  // twist: x * beta^p, canonical
+#pragma unroll
+  for (int k = 0; k < E; k += 2) {
+    if constexpr (!PF) w[k] = tw1[4 * k], w[k + 1] = tw1[4 * k + 4];
+    gl_mul2(x[k], x[k + 1], x[k], w[k], x[k + 1], w[k + 1]);
+  }
+  twisted_pass2_stage<0>(x, std::make_integer_sequence<int, 4>{});
+  twisted_pass2_stage<1>(x, std::make_integer_sequence<int, 4>{});
+  twisted_pass2_stage<2>(x, std::make_integer_sequence<int, 4>{});
+  twisted_pass2_stage<3>(x, std::make_integer_sequence<int, 4>{});
+  // exchange to pass 3's layout: positions 4(lane + 64q) + r
+#pragma unroll
+  for (int k = 0; k < E; k++) lds[b2 + 4 * k] = x[k];
+  const int ip = lane & 15;
+  const u64 w8 = twx[1088 + ip], w9a = twx[1104 + 2 * ip], w9b = twx[1105 + 2 * ip];
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) x[4 * q + r] = lds[b3 + 272 * q + r];
+  }
+  wave_sync();
+#pragma unroll
+  for (int q = 0; q < E / 4; q++) {
+    Bfly<Gl>::ct2_v(x[4 * q + 0], x[4 * q + 2], w8, x[4 * q + 1], x[4 * q + 3], w8);
+    Bfly<Gl>::ct2_v(x[4 * q + 0], x[4 * q + 1], w9a, x[4 * q + 2], x[4 * q + 3], w9b);
   }
 }
 

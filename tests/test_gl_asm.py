@@ -201,6 +201,18 @@ def test_mac_pair():
         assert out["Pb"] % P == (env["pb"] + mont(env["x"], env["kb"])) % P
 
 
+def test_mul_pair():
+    mul = parse_macros()["HB_GL_MUL2_ASM"]
+    rng = random.Random(4)
+    xs = samples(rng, 300)
+    ws = samples(rng, 300, canonical=True)
+    for i in range(len(xs)):
+        env = dict(xa=xs[i], wa=ws[i % len(ws)], xb=xs[-1 - i], wb=ws[(7 * i) % len(ws)])
+        out = run_macro(mul, env)
+        assert out["Ra"] == mont(env["xa"], env["wa"])  # canonical product
+        assert out["Rb"] == mont(env["xb"], env["wb"])
+
+
 def _pow2_cases(typ):
     """(s1, s2, exponent e of the multiplier 2^e mod P) for each shift of a type."""
     if typ == "A":
@@ -208,6 +220,53 @@ def _pow2_cases(typ):
     if typ == "BL":
         return [(j, 32 - j, j - 64) for j in range(1, 32)]
     return [(j - 32, 64 - j, j - 64) for j in range(33, 64)]
+
+
+@pytest.mark.parametrize("ta", ["A", "BL", "BH", "TN", "TC"])
+@pytest.mark.parametrize("tb", ["A", "BL", "BH", "TN", "TC"])
+def test_ct_pairs_with_unit_twiddle(ta, tb):
+    """CT pairs where a chain's twiddle is 1: TN takes any y word, TC a canonical y."""
+    if "T" not in ta[0] + tb[0]:
+        pytest.skip("no unit twiddle (test_pow2_pairs)")
+    mac = parse_macros()[f"HB_GL_CTP2_{ta}_{tb}_ASM"]
+    rng = random.Random(hash((ta, tb)) & 0xFFFF)
+    cases = {t: ([(0, 0, 0)] if t in ("TN", "TC") else _pow2_cases(t)) for t in (ta, tb)}
+    anyw, canw = samples(rng, 60), samples(rng, 60, canonical=True)
+    for i in range(max(len(cases[ta]), len(cases[tb])) * 3 + 60):
+        sa, sb = cases[ta][i % len(cases[ta])], cases[tb][(7 * i) % len(cases[tb])]
+        ya = (canw if ta == "TC" else anyw)[(3 * i + 1) % 60]
+        yb = (canw if tb == "TC" else anyw)[(11 * i + 3) % 60]
+        env = dict(xa=anyw[i % 60], ya=ya, xb=anyw[(5 * i + 2) % 60], yb=yb,
+                   SA1=sa[0], SA2=sa[1], SB1=sb[0], SB2=sb[1])
+        out = run_macro(mac, env)
+        for t, (_, _, e) in (("a", sa), ("b", sb)):
+            x, y = env["x" + t], env["y" + t]
+            m = pow(2, e % 192, P)
+            assert out["X" + t] % P == (x + y * m) % P, (t, e)
+            assert out["Y" + t] % P == (x - y * m) % P, (t, e)
+
+
+def test_unit_twiddle_mutation_is_caught():
+    """A TN chain that skipped the canonicalisation of y gives a wrong word for some y >= P
+    (the single EPS correction of x + y cannot absorb two wraps): the edge words catch it."""
+    lines, ops = parse_macros()["HB_GL_CTP2_TN_TN_ASM"]
+    bad = [ln for ln in lines if not ln.startswith("v_cndmask_b32") or ", -1," in ln or ", 0, -" in ln]
+    broken = (bad, ops)
+    found = False
+    for x in EDGE:
+        for y in EDGE:
+            env = dict(xa=x, ya=y, xb=x, yb=y, SA1=0, SA2=0, SB1=0, SB2=0)
+            try:
+                out = run_macro(broken, env)
+            except KeyError:
+                found = True
+                break
+            if out["Xa"] % P != (x + y) % P or out["Ya"] % P != (x - y) % P:
+                found = True
+                break
+        if found:
+            break
+    assert found
 
 
 @pytest.mark.parametrize("kind", ["CT", "GS"])
