@@ -299,6 +299,50 @@ def mac_chain(tag: str, base: int) -> Chain:
     return c
 
 
+def r4bytes_chain(tag: str, base: int) -> Chain:
+    """Stages 0 and 1 of the forward transform for one group of byte digits (a, b, c, d) at
+    registers (g, g + 4, g + 8, g + 12): a, b, c, d are byte g of the shared dwords w0..w3,
+    placed by v_perm_b32 (selector operands %<x><tag>: a, c16 = c << 16, b24 = b << 24,
+    d8 = d << 8, b8 = b << 8, d24 = d << 24) into the chain's own temporaries. With the stage
+    twiddles 2^48, -2^24 and -2^72 = 2^8 - 2^40 (mod P):
+      A = a + 2^48 c = (c16 : a),  C = a - 2^48 c + P = (~c16 : a + 1),
+      U = 2^24 b + 2^72 d = (d8 : b24) - (0 : d8),  V = 2^72 b + 2^24 d = (b8 : d24) - (0 : b8)
+      (both in [0, 2^48)); outputs y0 = A - U (+P on borrow), y1 = A + U (< 2^57),
+      y2 = C - V (C > 2^63 > V), y3 = C + V (+EPS on carry)."""
+    c = Chain(tag, base)
+    w0, w1, w2, w3 = "%w0", "%w1", "%w2", "%w3"
+    sa, sc, sbh, sdl, sbl, sdh = (f"%s{n}{tag}" for n in ("a", "c", "bh", "dl", "bl", "dh"))
+    c.add(f"v_perm_b32 {{W.lo}}, 0, {w1}, {{{sbh}}}", ["W.lo"], [w1, sbh])
+    c.add(f"v_perm_b32 {{W.hi}}, 0, {w3}, {{{sdl}}}", ["W.hi"], [w3, sdl])
+    c.add("v_sub_co_u32 {U.lo}, {S1}, {W.lo}, {W.hi}", ["U.lo", "S1"], ["W.lo", "W.hi"])
+    c.add("v_subb_co_u32 {U.hi}, {S1}, {W.hi}, 0, {S1}", ["U.hi", "S1"], ["W.hi", "S1"])
+    c.add(f"v_perm_b32 {{A.lo}}, 0, {w3}, {{{sdh}}}", ["A.lo"], [w3, sdh])
+    c.add(f"v_perm_b32 {{A.hi}}, 0, {w1}, {{{sbl}}}", ["A.hi"], [w1, sbl])
+    c.add("v_sub_co_u32 {V.lo}, {S2}, {A.lo}, {A.hi}", ["V.lo", "S2"], ["A.lo", "A.hi"])
+    c.add("v_subb_co_u32 {V.hi}, {S2}, {A.hi}, 0, {S2}", ["V.hi", "S2"], ["A.hi", "S2"])
+    c.add(f"v_perm_b32 {{T1}}, 0, {w0}, {{{sa}}}", ["T1"], [w0, sa])
+    c.add(f"v_perm_b32 {{T2}}, 0, {w2}, {{{sc}}}", ["T2"], [w2, sc])
+    c.add("v_add_u32 {Z.lo}, 1, {T1}", ["Z.lo"], ["T1"])
+    c.add("v_not_b32 {Z.hi}, {T2}", ["Z.hi"], ["T2"])
+    # y1 = A + U and y0 = A - U
+    c.add(f"v_add_co_u32 {{%Y1{tag}lo}}, {{S1}}, {{T1}}, {{U.lo}}", [f"%Y1{tag}lo", "S1"], ["T1", "U.lo"])
+    c.add(f"v_addc_co_u32 {{%Y1{tag}hi}}, {{S1}}, {{T2}}, {{U.hi}}, {{S1}}", [f"%Y1{tag}hi", "S1"],
+          ["T2", "U.hi", "S1"])
+    c.add("v_sub_co_u32 {W.lo}, {S2}, {T1}, {U.lo}", ["W.lo", "S2"], ["T1", "U.lo"])
+    c.add("v_subb_co_u32 {W.hi}, {S2}, {T2}, {U.hi}, {S2}", ["W.hi", "S2"], ["T2", "U.hi", "S2"])
+    c.add("v_cndmask_b32 {T1}, 0, -3, {S2}", ["T1"], ["S2"])
+    c.add(f"v_mad_i64_i32 {{%Y0{tag}}}, {{D}}, {{T1}}, {{%k55}}, {{W}}", [f"%Y0{tag}"], ["T1", "%k55", "W"])
+    # y2 = C - V and y3 = C + V
+    c.add(f"v_sub_co_u32 {{%Y2{tag}lo}}, {{S1}}, {{Z.lo}}, {{V.lo}}", [f"%Y2{tag}lo", "S1"], ["Z.lo", "V.lo"])
+    c.add(f"v_subb_co_u32 {{%Y2{tag}hi}}, {{S1}}, {{Z.hi}}, {{V.hi}}, {{S1}}", [f"%Y2{tag}hi", "S1"],
+          ["Z.hi", "V.hi", "S1"])
+    c.add("v_add_co_u32 {A.lo}, {S2}, {Z.lo}, {V.lo}", ["A.lo", "S2"], ["Z.lo", "V.lo"])
+    c.add("v_addc_co_u32 {A.hi}, {S2}, {Z.hi}, {V.hi}, {S2}", ["A.hi", "S2"], ["Z.hi", "V.hi", "S2"])
+    c.add("v_cndmask_b32 {T2}, 0, -1, {S2}", ["T2"], ["S2"])
+    c.add(f"v_mad_u64_u32 {{%Y3{tag}}}, {{D}}, {{T2}}, 1, {{A}}", [f"%Y3{tag}"], ["T2", "A"])
+    return c
+
+
 def mul_chain(tag: str, base: int) -> Chain:
     """r = montmul(x, w), canonical; x in [0, 2^64), w < P (Montgomery constant form)."""
     c = Chain(tag, base)
@@ -306,8 +350,10 @@ def mul_chain(tag: str, base: int) -> Chain:
     return c
 
 
-def schedule(chains: list[Chain]) -> list[str]:
-    """List-schedule the chains; returns instruction lines with physical registers."""
+def schedule(chains: list[Chain], early_clobber: bool = False) -> list[str]:
+    """List-schedule the chains; returns instruction lines with physical registers.
+    early_clobber: the outputs are early-clobber operands (never share a register with an
+    input), so they may be written before the last input read."""
     # nodes: (chain index, instr index)
     nodes = []
     for ci, c in enumerate(chains):
@@ -359,7 +405,7 @@ def schedule(chains: list[Chain]) -> list[str]:
             if uses_z(ins):
                 deps[(ci, j)] += [(p, 1) for p in prev]
     # outputs (%X/%Y/%P defs) only after every read of any input operand
-    in_readers = [n for n in nodes
+    in_readers = [] if early_clobber else [n for n in nodes
                   if any(u.startswith("%") and u != "%k55"
                          for u in chains[n[0]].ins[n[1]][2])]
     for n in nodes:
@@ -426,7 +472,7 @@ def schedule(chains: list[Chain]) -> list[str]:
     return lines
 
 
-def verify(lines: list[str], n_out: int, vgpr_inputs: set[int]) -> None:
+def verify(lines: list[str], n_out: int, vgpr_inputs: set[int], early_clobber: bool = False) -> None:
     """Static checks of one emitted block (operands already numbered):
     * a VALU write of an SGPR operand and a later VALU read of it as carry-in or mask
       are at least 3 instructions apart (two wait states on gfx950);
@@ -482,24 +528,26 @@ def verify(lines: list[str], n_out: int, vgpr_inputs: set[int]) -> None:
                 lo = int(vm.group(1))
                 hi = int(vm.group(2) or lo)
                 written.update(range(lo, hi + 1))
-    assert first_out_write is None or first_out_write > last_in_read, "output before input read"
+    assert early_clobber or first_out_write is None or first_out_write > last_in_read, \
+        "output before input read"
 
 
 def emit(name: str, chains: list[Chain], outputs: list[tuple[str, str, str]],
-         inputs: list[tuple[str, str, str]], doc: str) -> str:
-    """outputs/inputs: (symbol, constraint, C expression)."""
+         inputs: list[tuple[str, str, str]], doc: str, early_clobber: bool = False) -> str:
+    """outputs/inputs: (symbol, constraint, C expression). early_clobber: the outputs'
+    constraints are "=&v" and may be written before the last input read."""
     # the zero half of the zero-extension pair Z, for the chains that read Z at all
     def reads_z(c):
         """Whether the chain reads Z's zero half before writing all of Z."""
         for _, defs, uses in c.ins:
             if any(u.strip("{}") in ("Z", "Z.hi") for u in uses):
                 return True
-            if any(d.strip("{}") == "Z" for d in defs):
+            if any(d.strip("{}") in ("Z", "Z.hi") for d in defs):
                 return False
         return False
     zeros = ([TEMP_BASE + 9] if any(reads_z(c) for c in chains) else []) if SHARED_Z else \
         [c.base + 9 for c in chains if reads_z(c)]
-    lines = [f"v_mov_b32 v{z}, 0" for z in zeros] + schedule(chains)
+    lines = [f"v_mov_b32 v{z}, 0" for z in zeros] + schedule(chains, early_clobber)
     sg = []
     for c in chains:
         sg += [(f"s1{c.tag}", "=&s", f"s1{c.tag}"), (f"s2{c.tag}", "=&s", f"s2{c.tag}")]
@@ -516,7 +564,7 @@ def emit(name: str, chains: list[Chain], outputs: list[tuple[str, str, str]],
     numbered = [re.sub(r"%([A-Za-z][A-Za-z0-9]*)", lambda m: f"%{index[m.group(1)]}",
                        re.sub(r"\{%([A-Za-z0-9]+)\}", lambda m: f"%{index[m.group(1)]}", ln))
                 for ln in lines]
-    verify(numbered, len(outputs), {index[sym] for sym, c, _ in inputs if c == "v"})
+    verify(numbered, len(outputs), {index[sym] for sym, c, _ in inputs if c == "v"}, early_clobber)
     n_nop = sum(1 for ln in lines if ln.startswith("s_nop"))
     return (f"// {doc}\n// {len(lines)} instructions, {n_nop} of them s_nop, for "
             f"{len(chains)} chains.\n"
@@ -595,6 +643,23 @@ def main():
     inputs.append(("k55", "s", "0x55555555u"))
     parts.append(emit("mac2", [a, b], outputs, inputs,
                       "(pa, pb) += (x * ka, x * kb), keys in Montgomery constant form."))
+    a, b = r4bytes_chain("a", TEMP_BASE), r4bytes_chain("b", TEMP_BASE + 12)
+    outputs = []
+    for t in "ab":
+        outputs += [(f"Y0{t}", "=&v", f"Y0{t}"), (f"Y1{t}lo", "=&v", f"Y1{t}lo"),
+                    (f"Y1{t}hi", "=&v", f"Y1{t}hi"), (f"Y2{t}lo", "=&v", f"Y2{t}lo"),
+                    (f"Y2{t}hi", "=&v", f"Y2{t}hi"), (f"Y3{t}", "=&v", f"Y3{t}")]
+    inputs = [(f"w{m}", "v", f"w[{m}]") for m in range(4)]
+    for t, g in (("a", "G"), ("b", "(G + 1)")):
+        inputs += [(f"sa{t}", "s", f"0x0C0C0C00u | {g}"), (f"sc{t}", "s", f"0x0C000C0Cu | ({g} << 16)"),
+                   (f"sbh{t}", "s", f"0x000C0C0Cu | ({g} << 24)"),
+                   (f"sdl{t}", "s", f"0x0C0C000Cu | ({g} << 8)"),
+                   (f"sbl{t}", "s", f"0x0C0C000Cu | ({g} << 8)"),
+                   (f"sdh{t}", "s", f"0x000C0C0Cu | ({g} << 24)")]
+    inputs.append(("k55", "s", "0x55555555u"))
+    parts.append(emit("r4bytes2", [a, b], outputs, inputs,
+                      "Forward stages 0 and 1 of byte-digit groups G and G + 1 (r4bytes_chain): "
+                      "y0..y3 in [0, 2^64) from the digit dwords w[0..3].", early_clobber=True))
     a, b = mul_chain("a", TEMP_BASE), mul_chain("b", TEMP_BASE + 12)
     outputs = [("Ra", "=v", "Ra"), ("Rb", "=v", "Rb")]
     inputs = []

@@ -250,9 +250,7 @@ __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
     __builtin_amdgcn_sched_barrier(0);
     const u32 w[4] = {d.x, d.y, d.z, d.w};
     u64 x[E];
-#pragma unroll
-    for (int e = 0; e < E; e++) x[e] = (w[e >> 2] >> (8 * (e & 3))) & 0xFFu;
-    ntt_fwd_digits<PF>(x, lds, twv, lane);
+    ntt_fwd_bytes<PF>(x, w, lds, twv, lane);
     mac_row<Gl, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
     d = dn;
   }

@@ -624,21 +624,17 @@ __device__ __forceinline__ void twisted_pass2_stage(u64 (&x)[16], std::integer_s
   (twisted_pass2_pair<S, Ps>(x), ...);
 }
 
-// In: x[e] = digit word of position lane + 64e (< 2^16). Out: x[4q + r] = slot 4(lane + 64q) + r,
-// words in [0, 2^64). twx: the table above (normally an LDS copy). PF: issue the twist and
-// pass-3 table reads ahead of the exchanges (one wave per SIMD) instead of just before use.
+// Stages 2 and 3 (pass 1, compile-time shifts), the exchange, the twist, pass 2 and pass 3
+// of the twisted transform, from x after stages 0 and 1 (x[e] = position lane + 64e, any
+// words). Out: x[4q + r] = slot 4(lane + 64q) + r, words in [0, 2^64). PF: issue the twist
+// and pass-3 table reads ahead of the exchanges (one wave per SIMD) instead of just before
+// use.
 template <bool PF>
-__device__ __forceinline__ void ntt_fwd_digits(u64 (&x)[16], u64* lds, const u64* twx, int lane) {
+__device__ __forceinline__ void ntt_fwd_twisted_from2(u64 (&x)[16], u64* lds, const u64* twx,
+                                                      int lane) {
   constexpr int E = 16;
   const int b1 = lane, b2 = lds_base2(lane), b3 = lds_base3(lane);
-  // stage 0 exactly (inputs below 2^16; twiddle 2^48), then stages 1-3 at compile-time shifts
-#pragma unroll
-  for (int e = 0; e < E / 2; e++) {
-    const u32 a0 = (u32)x[e], t1 = (u32)x[e + E / 2] << 16;
-    x[e] = ((u64)t1 << 32) | a0;
-    x[e + E / 2] = ((u64)(~t1) << 32) | (a0 + 1u);
-  }
-  pass1_all<Gl, E, false, 1>(x, std::make_integer_sequence<int, 3>{});
+  pass1_all<Gl, E, false, 2>(x, std::make_integer_sequence<int, 2>{});
   // exchange to pass 2's layout: positions 64b + j0 + 4k, b = lane >> 2, j0 = lane & 3
 #pragma unroll
   for (int e = 0; e < E; e++) lds[b1 + 68 * e] = x[e];
@@ -679,6 +675,57 @@ __device__ __forceinline__ void ntt_fwd_digits(u64 (&x)[16], u64* lds, const u64
     Bfly<Gl>::ct2_v(x[4 * q + 0], x[4 * q + 1], w9a, x[4 * q + 2], x[4 * q + 3], w9b);
   }
 }
+
+// In: x[e] = digit word of position lane + 64e (< 2^16). Out: x[4q + r] = slot 4(lane + 64q) + r,
+// words in [0, 2^64). twx: the table above (normally an LDS copy).
+template <bool PF>
+__device__ __forceinline__ void ntt_fwd_digits(u64 (&x)[16], u64* lds, const u64* twx, int lane) {
+  constexpr int E = 16;
+  // stage 0 exactly (inputs below 2^16; twiddle 2^48), then stage 1 at compile-time shifts
+#pragma unroll
+  for (int e = 0; e < E / 2; e++) {
+    const u32 a0 = (u32)x[e], t1 = (u32)x[e + E / 2] << 16;
+    x[e] = ((u64)t1 << 32) | a0;
+    x[e + E / 2] = ((u64)(~t1) << 32) | (a0 + 1u);
+  }
+  pass1_all<Gl, E, false, 1>(x, std::make_integer_sequence<int, 1>{});
+  ntt_fwd_twisted_from2<PF>(x, lds, twx, lane);
+}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+// Stages 0 and 1 of the byte-digit groups G and G + 1 (HB_GL_R4BYTES2_ASM).
+template <u32 G>
+__device__ __forceinline__ void byte_stages01(u64 (&x)[16], const u32 (&w)[4]) {
+  u64 Y0a, Y3a, Y0b, Y3b, dummy, s1a, s2a, s1b, s2b;
+  u32 Y1alo, Y1ahi, Y2alo, Y2ahi, Y1blo, Y1bhi, Y2blo, Y2bhi;
+  HB_GL_R4BYTES2_ASM;
+  x[G] = Y0a, x[G + 4] = ((u64)Y1ahi << 32) | Y1alo;
+  x[G + 8] = ((u64)Y2ahi << 32) | Y2alo, x[G + 12] = Y3a;
+  x[G + 1] = Y0b, x[G + 5] = ((u64)Y1bhi << 32) | Y1blo;
+  x[G + 9] = ((u64)Y2bhi << 32) | Y2blo, x[G + 13] = Y3b;
+}
+
+// ntt_fwd_digits for byte digits held packed: w[m] byte g = the digit of position
+// lane + 64(4m + g) (store_digits8's layout). Stages 0 and 1 of each group of positions
+// lane + 64(g + {0, 4, 8, 12}) are formed from byte-placed dwords (v_perm_b32 selector bytes
+// 0-3 pick the second source's bytes, 0x0C a zero byte) inside HB_GL_R4BYTES2_ASM, with no
+// per-digit extraction.
+template <bool PF>
+__device__ __forceinline__ void ntt_fwd_bytes(u64 (&x)[16], const u32 (&w)[4], u64* lds,
+                                              const u64* twx, int lane) {
+  byte_stages01<0>(x, w);
+  byte_stages01<2>(x, w);
+  ntt_fwd_twisted_from2<PF>(x, lds, twx, lane);
+}
+#else
+// Host compilation pass only parses device code; this body is never compiled for a GPU.
+template <bool PF>
+__device__ inline void ntt_fwd_bytes(u64 (&x)[16], const u32 (&w)[4], u64* lds, const u64* twx,
+                                     int lane) {
+  for (int e = 0; e < 16; e++) x[e] = (w[e >> 2] >> (8 * (e & 3))) & 0xFFu;
+  ntt_fwd_digits<PF>(x, lds, twx, lane);
+}
+#endif
 
 // Inverse transform without the 1/N factor. In: slot layout of ntt_fwd_wave.
 // Out: x[e] = position lane + 64e, canonical. itwv = psi^-bitrev(k) for passes 3 and 2

@@ -38,7 +38,7 @@ def parse_macros():
 def c_eval(expr, env):
     if expr in env:
         return env[expr]
-    e = expr.replace("0x55555555u", "0x55555555")
+    e = re.sub(r"(0x[0-9A-Fa-f]+)u\b", r"\1", expr)
     e = re.sub(r"\(u32\)\(([^()]*)\)", r"u32(\1)", e)
     e = re.sub(r"\(u32\)(\w+)", r"u32(\1)", e)
     return eval(e, {"u32": lambda v: v & M32}, env)
@@ -124,6 +124,19 @@ class Lane:
             elif mn == "v_alignbit_b32":
                 v = (self.rd32(a[1]) << 32) | self.rd32(a[2])
                 self.wr32(a[0], v >> (self.rd32(a[3]) & 31))
+            elif mn == "v_perm_b32":
+                s0, s1, sel = self.rd32(a[1]), self.rd32(a[2]), self.rd32(a[3])
+                pool = [(s1 >> (8 * i)) & 0xFF for i in range(4)] + [(s0 >> (8 * i)) & 0xFF for i in range(4)]
+                r = 0
+                for i in range(4):
+                    b = (sel >> (8 * i)) & 0xFF
+                    assert b < 8 or b == 0x0C, hex(sel)
+                    r |= (pool[b] if b < 8 else 0) << (8 * i)
+                self.wr32(a[0], r)
+            elif mn == "v_not_b32":
+                self.wr32(a[0], ~self.rd32(a[1]))
+            elif mn == "v_add_u32":
+                self.wr32(a[0], self.rd32(a[1]) + self.rd32(a[2]))
             elif mn == "s_or_b64":
                 self.op[int(a[0][1:])] = self.bit(a[1]) | self.bit(a[2])
             else:
@@ -138,7 +151,7 @@ def run_macro(macro, env):
             vals[i] = c_eval(expr, env)
     lane = Lane(vals)
     lane.run(lines)
-    return {expr: lane.op[i] for i, (con, expr) in enumerate(ops) if con in ("=v",)}
+    return {expr: lane.op[i] for i, (con, expr) in enumerate(ops) if con in ("=v", "=&v")}
 
 
 def mont(y, c):
@@ -211,6 +224,33 @@ def test_mul_pair():
         out = run_macro(mul, env)
         assert out["Ra"] == mont(env["xa"], env["wa"])  # canonical product
         assert out["Rb"] == mont(env["xb"], env["wb"])
+
+
+def test_byte_digit_stages():
+    """HB_GL_R4BYTES2_ASM: forward stages 0 and 1 of a group of byte digits (a, b, c, d) at
+    registers (g, g+4, g+8, g+12), against the transform's own twiddles psi^512, psi^256,
+    psi^768 (psi of order 2048), on random and extreme bytes."""
+    mac = parse_macros()["HB_GL_R4BYTES2_ASM"]
+    psi = pow(7, (P - 1) // 2048, P)
+    w0, w1, w2 = pow(psi, 512, P), pow(psi, 256, P), pow(psi, 768, P)
+    rng = random.Random(5)
+    groups = [tuple(rng.randrange(256) for _ in range(4)) for _ in range(400)]
+    groups += [(a, b, c, d) for a in (0, 255) for b in (0, 255) for c in (0, 255) for d in (0, 255)]
+    for i in range(0, len(groups) - 1, 2):
+        G = (i // 2) % 3  # groups G and G + 1: bytes G, G + 1 of the dwords
+        w = [rng.randrange(2**32) for _ in range(4)]  # the other bytes are noise
+        for t, (a, b, c, d) in zip("ab", groups[i:i + 2]):
+            g = G + (t == "b")
+            for m, v in enumerate((a, b, c, d)):
+                w[m] = (w[m] & ~(0xFF << (8 * g))) | (v << (8 * g))
+        out = run_macro(mac, dict(w=w, G=G))
+        for t, (a, b, c, d) in zip("ab", groups[i:i + 2]):
+            A, C = (a + w0 * c) % P, (a - w0 * c) % P
+            B, D = (b + w0 * d) % P, (b - w0 * d) % P
+            want = [(A + w1 * B) % P, (A - w1 * B) % P, (C + w2 * D) % P, (C - w2 * D) % P]
+            got = [out[f"Y0{t}"], out[f"Y1{t}lo"] | (out[f"Y1{t}hi"] << 32),
+                   out[f"Y2{t}lo"] | (out[f"Y2{t}hi"] << 32), out[f"Y3{t}"]]
+            assert [g % P for g in got] == want, (a, b, c, d)
 
 
 def _pow2_cases(typ):
