@@ -256,6 +256,41 @@ __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
   }
 }
 
+// Latency mode's share of the Goldilocks 8-bit gadget: digits j0 and j0 + 1 of the rotated
+// component (`dst`, canonical words), keys from row j0 (`key`). Both digits are packed from
+// registers into ntt_fwd_bytes' layout (dword m byte g = the digit of word 4m + g) with four
+// byte permutes per group of four words, as store_digits8 does for all eight.
+template <bool PF>
+__device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
+                                                  const u64 (&dst)[16], u32 j0,
+                                                  const u64* __restrict__ key, u64* lds,
+                                                  const u64* twv, int lane) {
+  constexpr int N = 1024;
+  u32 d[4], dn[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    const u32 a0 = (u32)(dst[4 * m + 0] >> (8 * j0)), a1 = (u32)(dst[4 * m + 1] >> (8 * j0));
+    const u32 a2 = (u32)(dst[4 * m + 2] >> (8 * j0)), a3 = (u32)(dst[4 * m + 3] >> (8 * j0));
+    const u32 t01 = __builtin_amdgcn_perm(a1, a0, 0x05010400u);
+    const u32 t23 = __builtin_amdgcn_perm(a3, a2, 0x05010400u);
+    d[m] = __builtin_amdgcn_perm(t23, t01, 0x05040100u);
+    dn[m] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
+  }
+#pragma unroll 1
+  for (u32 j = 0; j < 2; j++) {
+    const u64* ka = key + (size_t)(2 * j) * N;
+    u64 ka0[4], kb0[4];
+    load4(ka + 4 * lane, ka0);
+    load4(ka + N + 4 * lane, kb0);
+    __builtin_amdgcn_sched_barrier(0);
+    u64 x[16];
+    ntt_fwd_bytes<PF>(x, d, lds, twv, lane);
+    mac_row<Gl, 16>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
+#pragma unroll
+    for (int m = 0; m < 4; m++) d[m] = dn[m];
+  }
+}
+
 // Wave-wide sum mod P of one value per lane.
 template <class F>
 __device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
@@ -521,9 +556,11 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
 // each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then
 // key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
-// window (at most 4 digits) fits 32 bits. Untraced semantics only.
+// window (at most 4 digits) fits 32 bits; DIG8 (Goldilocks with 8 digits of 8 bits, 8 waves):
+// a wave's two byte digits go through the byte-digit transform (gadget_mac_bytes2).
+// Untraced semantics only.
 // ----------------------------------------------------------------------------
-template <class F, int WAVES>
+template <class F, int WAVES, bool DIG8 = false>
 __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      typename DF<F>::W* lds, int ldsw,
                                                      unsigned short* ms,
@@ -531,6 +568,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      const typename DF<F>::W* twl,
                                                      const typename DF<F>::W* itwl) {
   static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per gate");
+  static_assert(!DIG8 || (F::IS_GOLDILOCKS && WAVES == 8), "two byte digits per wave");
   using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
@@ -569,7 +607,9 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       rotate_sub<F, E>(comp ? accB : accA, r, false, lane, dst);
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
       const W* kj = key + (size_t)j0 * 2 * N;
-      if constexpr (WAVES == 8) {
+      if constexpr (DIG8) {
+        gadget_mac_bytes2<PF>(pa, pb, dst, j0, kj, lds, twl, lane);
+      } else if constexpr (WAVES == 8) {
         u32 win[E];
 #pragma unroll
         for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
@@ -702,7 +742,7 @@ void bootstrap_kernel(BootArgs A) {
 }
 
 // Latency-mode launch: one gate per workgroup of WAVES wavefronts (bootstrap_gate_split).
-template <class F, int WAVES>
+template <class F, int WAVES, bool DIG8>
 __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[WAVES][1024 + 64];
@@ -711,8 +751,8 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   __shared__ W itwl[1024];
   __shared__ W part[4 * 2 * 1024];
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
-  bootstrap_gate_split<F, WAVES>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms, part,
-                                 twl, itwl);
+  bootstrap_gate_split<F, WAVES, DIG8>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
+                                       part, twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -800,7 +840,10 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   const int w = threadIdx.x >> 6;
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
-    bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
+    if (WAVES == 8 && dig8_set(A))
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
+    else
+      bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
   } else {
     u32* t = reinterpret_cast<u32*>(twl);
     u32* it = reinterpret_cast<u32*>(itwl);
@@ -1007,14 +1050,16 @@ hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int wave
   if (waves != 4 && waves != 8) return hipErrorInvalidValue;
   if (waves == 4) {
     if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4, false>), dim3(a.count), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4, false>), dim3(a.count), dim3(256), 0, s, a);
   } else {
-    if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8>), dim3(a.count), dim3(512), 0, s, a);
+    if (is_goldilocks && dig8_set(a))
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, true>), dim3(a.count), dim3(512), 0, s, a);
+    else if (is_goldilocks)
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, false>), dim3(a.count), dim3(512), 0, s, a);
     else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8>), dim3(a.count), dim3(512), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8, false>), dim3(a.count), dim3(512), 0, s, a);
   }
   return hipGetLastError();
 }
