@@ -558,6 +558,13 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
 // window (at most 4 digits) fits 32 bits; DIG8 (Goldilocks with 8 digits of 8 bits, 8 waves):
 // a wave's two byte digits go through the byte-digit transform (gadget_mac_bytes2).
+// The accumulator stays in LDS for the whole gate (accA, accB: N words each), so a round's
+// last barrier waits for LDS stores only and the next rotation reads LDS. With 8 waves it
+// shares the partial slots (accA = part, the a half of slot 0; accB = part + 2N, the a half
+// of slot 1), whose a halves are written only after the first barrier of a round (waves
+// 4-7 hand their a partials over in their own exchange buffers and their b partials in the
+// b halves), and each wave keeps the four accumulator words it updates at the end of the
+// round from its rotation. With 4 waves it has its own words.
 // Untraced semantics only.
 // ----------------------------------------------------------------------------
 template <class F, int WAVES, bool DIG8 = false>
@@ -565,6 +572,8 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      typename DF<F>::W* lds, int ldsw,
                                                      unsigned short* ms,
                                                      typename DF<F>::W* part,
+                                                     typename DF<F>::W* accA,
+                                                     typename DF<F>::W* accB,
                                                      const typename DF<F>::W* twl,
                                                      const typename DF<F>::W* itwl) {
   static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per gate");
@@ -581,8 +590,6 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   const int comp = wave / Q, q = wave % Q;
   const u32 n = A.n;
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
-  W* accA = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
-  W* accB = accA + N;
 
   gate_modswitch<F>(A, g, tid, 64 * WAVES, ms);
   __syncthreads();
@@ -593,7 +600,8 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   // this wave's digits: [j0, j1) of its component's gadget
   const u32 j0 = (u32)q * A.br_len / Q, j1 = (u32)(q + 1) * A.br_len / Q;
   // partial slots [4][a, b][e][lane]: wave w < 4 owns slot w (with 8 waves, wave w + 4
-  // hands its partial to wave w through the same slot first)
+  // hands its a partial to wave w through its own exchange buffer and its b partial
+  // through the b half of slot w first)
   W* mine = part + (size_t)(wave & 3) * 2 * N;
   for (u32 i = 0; i < n; i++) {
     const int r = ms[i];
@@ -602,9 +610,17 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
     W pa[E], pb[E];
 #pragma unroll
     for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+    // 8 waves: accumulator words 64 q + lane + 256 k of component `comp`, which this wave
+    // updates at the end of the round (after the slots holding them have been reused)
+    W accq[4];
     {
       W dst[E];
-      rotate_sub<F, E>(comp ? accB : accA, r, false, lane, dst);
+      const W* ac = comp ? accB : accA;
+      rotate_sub<F, E>(ac, r, false, lane, dst);
+      if constexpr (WAVES == 8) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) accq[k] = ac[64 * q + lane + 256 * k];
+      }
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
       const W* kj = key + (size_t)j0 * 2 * N;
       if constexpr (DIG8) {
@@ -626,15 +642,16 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       if (wave >= 4) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
-          mine[e * 64 + lane] = pa[e];
+          lds[e * 64 + lane] = pa[e];
           mine[N + e * 64 + lane] = pb[e];
         }
       }
       __syncthreads();
       if (wave < 4) {
+        const W* pard = lds + (ptrdiff_t)4 * ldsw;  // wave + 4's exchange buffer
 #pragma unroll
         for (int e = 0; e < E; e++) {
-          pa[e] = DF<F>::add(pa[e], mine[e * 64 + lane]);
+          pa[e] = DF<F>::add(pa[e], pard[e * 64 + lane]);
           pb[e] = DF<F>::add(pb[e], mine[N + e * 64 + lane]);
         }
       }
@@ -668,7 +685,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int pos = 64 * qw + lane + 256 * k;
-        acc[pos] = F::add(acc[pos], x[k]);
+        acc[pos] = F::add(accq[k], x[k]);
       }
     } else {
       // wave w finishes quarter w of both halves: the a quarter in words [0, 320) of its
@@ -750,9 +767,12 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   __shared__ W twl[kTwxWords];
   __shared__ W itwl[1024];
   __shared__ W part[4 * 2 * 1024];
+  __shared__ W accl[WAVES == 4 ? 2 * 1024 : 1];
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
+  W* const aa = WAVES == 4 ? accl : part;
+  W* const ab = WAVES == 4 ? accl + 1024 : part + 2 * 1024;
   bootstrap_gate_split<F, WAVES, DIG8>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
-                                       part, twl, itwl);
+                                       part, aa, ab, twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -832,24 +852,31 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   __shared__ u64 twl[kTwxWords];
   __shared__ u64 itwl[1024];
   __shared__ u64 part[4 * 2 * 1024];
+  __shared__ u64 accl[WAVES == 4 ? 2 * 1024 : 1];
   const u32 b = blockIdx.x;
   int p = 0;
   while (p + 1 < nparts && b >= parts[p + 1].first_block) p++;
   const BootArgs A = parts[p];
   const int g = (int)(b - A.first_block);
   const int w = threadIdx.x >> 6;
+  // the accumulator's LDS words (bootstrap_gate_split), in the set's word size
+  u64* const aa = WAVES == 4 ? accl : part;
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
+    u64* const ab = WAVES == 4 ? aa + 1024 : aa + 2 * 1024;
     if (WAVES == 8 && dig8_set(A))
-      bootstrap_gate_split<Gl, WAVES, WAVES == 8>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8>(A, g, lds[w], 1024 + 64, ms, part, aa, ab, twl,
+                                                  itwl);
     else
-      bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, twl, itwl);
+      bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, aa, ab, twl, itwl);
   } else {
     u32* t = reinterpret_cast<u32*>(twl);
     u32* it = reinterpret_cast<u32*>(itwl);
     stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64 * WAVES);
+    u32* const ba = reinterpret_cast<u32*>(aa);
+    u32* const bb = WAVES == 4 ? ba + 1024 : ba + 2 * 1024;
     bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64), ms,
-                                    reinterpret_cast<u32*>(part), t, it);
+                                    reinterpret_cast<u32*>(part), ba, bb, t, it);
   }
 }
 
