@@ -152,27 +152,30 @@ __device__ __forceinline__ void digit_ntt(typename DF<F>::W (&x)[E], typename DF
 // values, digit_j = (v >> j*bits) & mask. twv: digit_ntt's forward table (LDS copy). S: the source word type (a field word, or u32 for a window of at most 32 bits of
 // digits). A: the accumulator type (field words for Goldilocks, lazy 64-bit sums for
 // BabyBear, folded after every second row: the transform's words are below 2P, so
-// 2 (2P-1)(P-1) + 2^60 < 2^64).
+// 2 (2P-1)(P-1) + 2^60 < 2^64). swap: pa takes the b products and pb the a products.
 template <class F, int E, bool PF, class S, class A>
 __device__ __forceinline__ void gadget_rows(A (&pa)[E], A (&pb)[E], const S (&src)[E], u32 len,
                                             u32 bits, const typename DF<F>::W* __restrict__ key,
                                             typename DF<F>::W* lds,
-                                            const typename DF<F>::W* twv, int lane) {
+                                            const typename DF<F>::W* twv, int lane,
+                                            bool swap = false) {
   using W = typename DF<F>::W;
   constexpr int N = 64 * E;
   const W mask = (W)((1u << bits) - 1u);
+  const int oa = swap ? N : 0, ob = swap ? 0 : N;
   for (u32 j = 0; j < len; j++) {
-    const W* ka = key + (size_t)(2 * j) * N;
+    const W* ka = key + (size_t)(2 * j) * N + oa;
+    const W* kb = key + (size_t)(2 * j) * N + ob;
     W ka0[4], kb0[4];
     load4(ka + 4 * lane, ka0);
-    load4(ka + N + 4 * lane, kb0);
+    load4(kb + 4 * lane, kb0);
     __builtin_amdgcn_sched_barrier(0);
     W x[E];
     const u32 sh = j * bits;
 #pragma unroll
     for (int e = 0; e < E; e++) x[e] = (W)(src[e] >> sh) & mask;
     digit_ntt<F, E, PF>(x, lds, twv, lane);  // digits < 2^bits <= 2^16
-    mac_row<F, E>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
+    mac_row<F, E>(pa, pb, x, ka, kb, ka0, kb0, lane);
     if constexpr (!F::IS_GOLDILOCKS) {
       if ((j & 1) == 1 && j + 1 < len) {
 #pragma unroll
@@ -188,14 +191,15 @@ __device__ __forceinline__ void gadget_mac(typename DF<F>::W (&pa)[E], typename 
                                            const S (&src)[E], u32 len, u32 bits,
                                            const typename DF<F>::W* __restrict__ key,
                                            typename DF<F>::W* lds,
-                                           const typename DF<F>::W* twv, int lane) {
+                                           const typename DF<F>::W* twv, int lane,
+                                           bool swap = false) {
   if constexpr (F::IS_GOLDILOCKS) {
-    gadget_rows<F, E, PF>(pa, pb, src, len, bits, key, lds, twv, lane);
+    gadget_rows<F, E, PF>(pa, pb, src, len, bits, key, lds, twv, lane, swap);
   } else {
     u64 sa[E], sb[E];
 #pragma unroll
     for (int e = 0; e < E; e++) sa[e] = DF<F>::lazy_in(pa[e]), sb[e] = DF<F>::lazy_in(pb[e]);
-    gadget_rows<F, E, PF>(sa, sb, src, len, bits, key, lds, twv, lane);
+    gadget_rows<F, E, PF>(sa, sb, src, len, bits, key, lds, twv, lane, swap);
 #pragma unroll
     for (int e = 0; e < E; e++) pa[e] = DF<F>::lazy_out(sa[e]), pb[e] = DF<F>::lazy_out(sb[e]);
   }
@@ -259,13 +263,15 @@ __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
 // Latency mode's share of the Goldilocks 8-bit gadget: digits j0 and j0 + 1 of the rotated
 // component (`dst`, canonical words), keys from row j0 (`key`). Both digits are packed from
 // registers into ntt_fwd_bytes' layout (dword m byte g = the digit of word 4m + g) with four
-// byte permutes per group of four words, as store_digits8 does for all eight.
+// byte permutes per group of four words, as store_digits8 does for all eight. swap: pa
+// takes the b products and pb the a products.
 template <bool PF>
 __device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
                                                   const u64 (&dst)[16], u32 j0,
                                                   const u64* __restrict__ key, u64* lds,
-                                                  const u64* twv, int lane) {
+                                                  const u64* twv, int lane, bool swap) {
   constexpr int N = 1024;
+  const int oa = swap ? N : 0, ob = swap ? 0 : N;
   u32 d[4], dn[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) {
@@ -278,14 +284,15 @@ __device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
   }
 #pragma unroll 1
   for (u32 j = 0; j < 2; j++) {
-    const u64* ka = key + (size_t)(2 * j) * N;
+    const u64* ka = key + (size_t)(2 * j) * N + oa;
+    const u64* kb = key + (size_t)(2 * j) * N + ob;
     u64 ka0[4], kb0[4];
     load4(ka + 4 * lane, ka0);
-    load4(ka + N + 4 * lane, kb0);
+    load4(kb + 4 * lane, kb0);
     __builtin_amdgcn_sched_barrier(0);
     u64 x[16];
     ntt_fwd_bytes<PF>(x, d, lds, twv, lane);
-    mac_row<Gl, 16>(pa, pb, x, ka, ka + N, ka0, kb0, lane);
+    mac_row<Gl, 16>(pa, pb, x, ka, kb, ka0, kb0, lane);
 #pragma unroll
     for (int m = 0; m < 4; m++) d[m] = dn[m];
   }
@@ -550,8 +557,8 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // Latency mode: one gate per workgroup of WAVES = 4 or 8 wavefronts. Wave w takes component
 // c = w / Q of the rotated accumulator (Q = WAVES / 2 waves per component) and the q = w % Q
 // share of its gadget digits, [q l / Q, (q + 1) l / Q), accumulating a partial NTT-domain
-// product. The partials meet in LDS (with 8 waves, waves 4-7 first hand theirs to waves
-// 0-3). Then every wave finishes quarters of the result (sum of the partials, inverse
+// product. The partials meet in LDS (with 8 waves, waves w and w + 4 first add theirs
+// pairwise, a half each). Then every wave finishes quarters of the result (sum of the partials, inverse
 // NTT, accumulator add) with ntt_inv_quarter / ntt_inv_cross, exchanging through the
 // waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
 // each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then
@@ -561,10 +568,10 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // The accumulator stays in LDS for the whole gate (accA, accB: N words each), so a round's
 // last barrier waits for LDS stores only and the next rotation reads LDS. With 8 waves it
 // shares the partial slots (accA = part, the a half of slot 0; accB = part + 2N, the a half
-// of slot 1), whose a halves are written only after the first barrier of a round (waves
-// 4-7 hand their a partials over in their own exchange buffers and their b partials in the
-// b halves), and each wave keeps the four accumulator words it updates at the end of the
-// round from its rotation. With 4 waves it has its own words.
+// of slot 1), which are written only after the first barrier of a round (the partials of
+// waves w and w + 4 meet through their exchange buffers first), and each wave keeps the
+// four accumulator words it updates at the end of the round from its rotation. With 4
+// waves it has its own words.
 // Untraced semantics only.
 // ----------------------------------------------------------------------------
 template <class F, int WAVES, bool DIG8 = false>
@@ -599,9 +606,9 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
   // this wave's digits: [j0, j1) of its component's gadget
   const u32 j0 = (u32)q * A.br_len / Q, j1 = (u32)(q + 1) * A.br_len / Q;
-  // partial slots [4][a, b][e][lane]: wave w < 4 owns slot w (with 8 waves, wave w + 4
-  // hands its a partial to wave w through its own exchange buffer and its b partial
-  // through the b half of slot w first)
+  // partial slots [4][a, b][e][lane]: with 4 waves wave w stores slot w; with 8 waves
+  // wave w < 4 the a half of slot w and wave w + 4 its b half, each the sum of both waves'
+  // partials
   W* mine = part + (size_t)(wave & 3) * 2 * N;
   for (u32 i = 0; i < n; i++) {
     const int r = ms[i];
@@ -624,12 +631,12 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
       const W* kj = key + (size_t)j0 * 2 * N;
       if constexpr (DIG8) {
-        gadget_mac_bytes2<PF>(pa, pb, dst, j0, kj, lds, twl, lane);
+        gadget_mac_bytes2<PF>(pa, pb, dst, j0, kj, lds, twl, lane, wave >= 4);
       } else if constexpr (WAVES == 8) {
         u32 win[E];
 #pragma unroll
         for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
-        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, twl, lane);
+        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, twl, lane, wave >= 4);
       } else {
         if (j0 > 0) {
 #pragma unroll
@@ -639,25 +646,18 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       }
     }
     if constexpr (WAVES == 8) {
-      if (wave >= 4) {
+      // waves w and w + 4 meet in slot w: pa holds the half a wave keeps (the a products
+      // for waves 0-3, the b products for waves 4-7, whose keys were swapped) and pb the
+      // half it hands over in its own exchange buffer
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          lds[e * 64 + lane] = pa[e];
-          mine[N + e * 64 + lane] = pb[e];
-        }
-      }
+      for (int e = 0; e < E; e++) lds[e * 64 + lane] = pb[e];
       __syncthreads();
-      if (wave < 4) {
-        const W* pard = lds + (ptrdiff_t)4 * ldsw;  // wave + 4's exchange buffer
+      const W* take = lds + (ptrdiff_t)(wave < 4 ? 4 : -4) * ldsw;
+      W* put = mine + (wave < 4 ? 0 : N);
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-          pa[e] = DF<F>::add(pa[e], pard[e * 64 + lane]);
-          pb[e] = DF<F>::add(pb[e], mine[N + e * 64 + lane]);
-        }
-      }
-    }
-    // partial products to LDS, [poly][e][lane] (each lane rewrites only what it read)
-    if (wave < 4) {
+      for (int e = 0; e < E; e++) put[e * 64 + lane] = DF<F>::add(pa[e], take[e * 64 + lane]);
+    } else {
+      // partial products to LDS, [poly][e][lane]
 #pragma unroll
       for (int e = 0; e < E; e++) {
         mine[e * 64 + lane] = pa[e];
