@@ -415,8 +415,23 @@ __device__ __forceinline__ void rotate_sub(const typename DF<F>::W* acc, int r, 
   }
 }
 
+// rotate_sub (untraced) on a component held as cn = [C | M], C[k] = canon(acc[k]) and
+// M[k] = canon(neg_raw(acc[k])), 2N words: position k reads cn[(k - r) mod 2N], which is
+// M[k - r mod N] when the monomial's sign flips it (k < r for r <= N, k >= r - N above) and
+// C[k - r mod N] otherwise.
+template <class F, int E>
+__device__ __forceinline__ void rotate_sub_cn(const typename DF<F>::W* cn, int r, int lane,
+                                              typename DF<F>::W (&dst)[E]) {
+  constexpr int N = 64 * E;
+  const int t0 = lane - r;
+#pragma unroll
+  for (int e = 0; e < E; e++)
+    dst[e] = F::sub_canon(cn[(t0 + 64 * e) & (2 * N - 1)], cn[lane + 64 * e]);
+}
+
 // Step 4 by one wave: acc.b[0] += Q/8 (evaluate.rs:52), key switching and sample extraction.
-template <class F, bool PF = true>
+// NEGA: accA holds canon(neg_raw(acc.a)) (the M half of rotate_sub_cn's layout).
+template <class F, bool PF = true, bool NEGA = false>
 __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
                                                   const typename DF<F>::W* accA,
                                                   const typename DF<F>::W* accB,
@@ -434,7 +449,8 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int k = lane + 64 * e;
-    src[e] = F::canon(F::neg_raw(accA[k]));  // polynomial.neg_assign() (rlwe.rs:1159)
+    // polynomial.neg_assign() (rlwe.rs:1159)
+    src[e] = NEGA ? accA[k] : F::canon(F::neg_raw(accA[k]));
   }
   W pa[E], pb[E];
 #pragma unroll
@@ -565,13 +581,13 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
 // window (at most 4 digits) fits 32 bits; DIG8 (Goldilocks with 8 digits of 8 bits, 8 waves):
 // a wave's two byte digits go through the byte-digit transform (gadget_mac_bytes2).
-// The accumulator stays in LDS for the whole gate (accA, accB: N words each), so a round's
-// last barrier waits for LDS stores only and the next rotation reads LDS. With 8 waves it
-// shares the partial slots (accA = part, the a half of slot 0; accB = part + 2N, the a half
-// of slot 1), which are written only after the first barrier of a round (the partials of
-// waves w and w + 4 meet through their exchange buffers first), and each wave keeps the
-// four accumulator words it updates at the end of the round from its rotation. With 4
-// waves it has its own words.
+// The accumulator stays in LDS for the whole gate, so a round's last barrier waits for LDS
+// stores only and the next rotation reads LDS. With 8 waves each wave keeps the raw words
+// of the 16 positions it updates in registers for the whole gate, and LDS holds every
+// component as rotate_sub_cn's [C | M] (accA = part, slot 0; accB = part + 2N, slot 1;
+// the slots are written only after the first barrier of a round, the partials of waves w
+// and w + 4 meeting through their exchange buffers first). With 4 waves the raw words
+// (accA, accB: N words each) have their own LDS.
 // Untraced semantics only.
 // ----------------------------------------------------------------------------
 template <class F, int WAVES, bool DIG8 = false>
@@ -602,6 +618,20 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   __syncthreads();
   gate_lut_init<F>(A, tid, 64 * WAVES, ms, accA, accB);
   __syncthreads();
+  // 8 waves: the raw words of positions 64 q + lane + 256 k of component `comp`, which this
+  // wave updates at the end of every round, and the [C | M] form of the component
+  W raw[4];
+  if constexpr (WAVES == 8) {
+    W* ac = comp ? accB : accA;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int pos = 64 * q + lane + 256 * k;
+      raw[k] = ac[pos];
+      ac[pos] = F::canon(raw[k]);
+      ac[N + pos] = F::canon(F::neg_raw(raw[k]));
+    }
+    __syncthreads();
+  }
 
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
   // this wave's digits: [j0, j1) of its component's gadget
@@ -617,17 +647,13 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
     W pa[E], pb[E];
 #pragma unroll
     for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
-    // 8 waves: accumulator words 64 q + lane + 256 k of component `comp`, which this wave
-    // updates at the end of the round (after the slots holding them have been reused)
-    W accq[4];
     {
       W dst[E];
       const W* ac = comp ? accB : accA;
-      rotate_sub<F, E>(ac, r, false, lane, dst);
-      if constexpr (WAVES == 8) {
-#pragma unroll
-        for (int k = 0; k < 4; k++) accq[k] = ac[64 * q + lane + 256 * k];
-      }
+      if constexpr (WAVES == 8)
+        rotate_sub_cn<F, E>(ac, r, lane, dst);
+      else
+        rotate_sub<F, E>(ac, r, false, lane, dst);
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
       const W* kj = key + (size_t)j0 * 2 * N;
       if constexpr (DIG8) {
@@ -685,7 +711,9 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int pos = 64 * qw + lane + 256 * k;
-        acc[pos] = F::add(accq[k], x[k]);
+        raw[k] = F::add(raw[k], x[k]);
+        acc[pos] = F::canon(raw[k]);
+        acc[N + pos] = F::canon(F::neg_raw(raw[k]));
       }
     } else {
       // wave w finishes quarter w of both halves: the a quarter in words [0, 320) of its
@@ -717,7 +745,13 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
     }
     __syncthreads();
   }
-  if (wave == 0) keyswitch_extract<F, PF>(A, g, accA, accB, lds, twl, itwl, lane);
+  if (wave == 0) {
+    // b0 = acc.b[0] + Q/8 is canonicalised before use, so C[0] serves for the raw word
+    if constexpr (WAVES == 8)
+      keyswitch_extract<F, PF, true>(A, g, accA + N, accB, lds, twl, itwl, lane);
+    else
+      keyswitch_extract<F, PF>(A, g, accA, accB, lds, twl, itwl, lane);
+  }
 }
 
 // Copies the forward and inverse twiddle tables (N words each; the Goldilocks forward table
