@@ -410,8 +410,15 @@ __device__ __forceinline__ void rotate_sub(const typename DF<F>::W* acc, int r, 
       src = neg ? k - rr : k - rr + N;
     }
     W v = acc[src];
-    if (neg) v = traced ? F::neg_raw(F::canon(v)) : F::neg_raw(v);
-    dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
+    if constexpr (F::IS_GOLDILOCKS) {
+      // untraced: P - v (mod 2^64) is neg_raw(v) for v != 0 and P = 0 (mod P) for v = 0;
+      // the raw subtraction keeps the residue of any words, so one canonicalisation ends
+      if (neg) v = (u64)F::P - (traced ? F::canon(v) : v);
+      dst[e] = F::canon(F::sub(v, acc[k]));
+    } else {
+      if (neg) v = traced ? F::neg_raw(F::canon(v)) : F::neg_raw(v);
+      dst[e] = F::sub_canon(F::canon(v), F::canon(acc[k]));
+    }
   }
 }
 
