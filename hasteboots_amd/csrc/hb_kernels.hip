@@ -9,8 +9,8 @@
 //   KSK  [j < ks_len][p][N slots], same scaling.
 //   Slot order is the forward-NTT output order (bit-reversed evaluation order).
 //   Per-gate scratch (kScratchWordsPerGate words): the accumulator RLWE, 2N words, raw
-//   Goldilocks words exactly as the reference's add_reduce produces them, then N words
-//   for the byte digits of the rotated accumulator (store_digits8).
+//   Goldilocks words exactly as the reference's add_reduce produces them, then up to 2N
+//   words for the byte digits of the rotated accumulator (store_digits8, store_digits5).
 #include "hb_kernels.hpp"
 #include "hb_ntt.hpp"
 
@@ -234,9 +234,35 @@ __device__ __forceinline__ void store_digits8(const u64 (&v)[16], uint4* dig, in
   for (int j = 0; j < 8; j++) dig[64 * j + lane] = make_uint4(d[j][0], d[j][1], d[j][2], d[j][3]);
 }
 
-// gadget_mac for the Goldilocks blind-rotation gadget (8-bit digits), the digits read from
-// their byte-transposed copy (store_digits8): one 16-byte load per digit, issued a digit
-// ahead, in place of a 16-word source held in registers through the digit loop.
+// Digit j of four words (a0..a3 = the words shifted right by j * bits, low 32 bits) as one
+// dword of bytes (a0, a1, a2, a3) & mask: two byte permutes gather the low bytes pairwise
+// (a0.b0, a1.b0, a0.b1, a1.b1), a third the four low bytes.
+__device__ __forceinline__ u32 pack_low_bytes(u32 a0, u32 a1, u32 a2, u32 a3, u32 mask4) {
+  const u32 t01 = __builtin_amdgcn_perm(a1, a0, 0x05010400u);
+  const u32 t23 = __builtin_amdgcn_perm(a3, a2, 0x05010400u);
+  return __builtin_amdgcn_perm(t23, t01, 0x05040100u) & mask4;
+}
+
+// store_digits8's layout for a gadget of 13 digits of 5 bits (decompose_lsb_bits_inplace
+// with bits = 5, native_polynomial.rs:340-348; digit 12 holds bits 60-63): one byte per
+// digit, so the digit loop runs the byte-digit transform.
+__device__ __forceinline__ void store_digits5(const u64 (&v)[16], uint4* dig, int lane) {
+#pragma unroll
+  for (int j = 0; j < 13; j++) {
+    u32 d[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      d[m] = pack_low_bytes((u32)(v[4 * m] >> (5 * j)), (u32)(v[4 * m + 1] >> (5 * j)),
+                            (u32)(v[4 * m + 2] >> (5 * j)), (u32)(v[4 * m + 3] >> (5 * j)),
+                            0x1F1F1F1Fu);
+    dig[64 * j + lane] = make_uint4(d[0], d[1], d[2], d[3]);
+  }
+}
+
+// gadget_mac for a Goldilocks blind-rotation gadget of byte-sized digits, the digits read
+// from their byte-transposed copy (store_digits8, store_digits5): one 16-byte load per
+// digit, issued a digit ahead, in place of a 16-word source held in registers through the
+// digit loop.
 template <int E, bool PF>
 __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
                                                 const uint4* __restrict__ dig, u32 len,
@@ -298,6 +324,38 @@ __device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
   }
 }
 
+// Latency mode's share of a Goldilocks gadget of BITS-bit digits (BITS < 8): `len` digits
+// of the window `win` (the rotated component shifted right to the wave's first digit, keys
+// from that digit's row), each packed from registers into ntt_fwd_bytes' layout. swap: as
+// gadget_mac_bytes2.
+template <int BITS, bool PF>
+__device__ __forceinline__ void gadget_mac_window_bytes(u64 (&pa)[16], u64 (&pb)[16],
+                                                        const u32 (&win)[16], u32 len,
+                                                        const u64* __restrict__ key, u64* lds,
+                                                        const u64* twv, int lane, bool swap) {
+  constexpr int N = 1024;
+  constexpr u32 mask4 = ((1u << BITS) - 1u) * 0x01010101u;
+  const int oa = swap ? N : 0, ob = swap ? 0 : N;
+#pragma unroll 1
+  for (u32 j = 0; j < len; j++) {
+    const u64* ka = key + (size_t)(2 * j) * N + oa;
+    const u64* kb = key + (size_t)(2 * j) * N + ob;
+    u64 ka0[4], kb0[4];
+    load4(ka + 4 * lane, ka0);
+    load4(kb + 4 * lane, kb0);
+    __builtin_amdgcn_sched_barrier(0);
+    const u32 sh = j * BITS;
+    u32 d[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      d[m] = pack_low_bytes(win[4 * m] >> sh, win[4 * m + 1] >> sh, win[4 * m + 2] >> sh,
+                            win[4 * m + 3] >> sh, mask4);
+    u64 x[16];
+    ntt_fwd_bytes<PF>(x, d, lds, twv, lane);
+    mac_row<Gl, 16>(pa, pb, x, ka, kb, ka0, kb0, lane);
+  }
+}
+
 // Wave-wide sum mod P of one value per lane.
 template <class F>
 __device__ __forceinline__ typename DF<F>::W wave_sum(typename DF<F>::W v) {
@@ -319,9 +377,13 @@ __device__ __forceinline__ void gate_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Goldilocks sets whose blind-rotation gadget is 8 digits of 8 bits (the byte-digit path).
-__host__ __device__ inline bool dig8_set(const BootArgs& A) {
-  return A.is_goldilocks && A.br_bits == 8 && A.br_len == 8;
+// Goldilocks sets whose blind-rotation digits take the byte-digit path: 8 digits of 8 bits
+// (8) or 13 digits of 5 bits (5); 0 for any other gadget.
+__host__ __device__ inline int dig_bits(const BootArgs& A) {
+  if (!A.is_goldilocks) return 0;
+  if (A.br_bits == 8 && A.br_len == 8) return 8;
+  if (A.br_bits == 5 && A.br_len == 13) return 5;
+  return 0;
 }
 
 // ----------------------------------------------------------------------------
@@ -490,11 +552,11 @@ __device__ __forceinline__ void keyswitch_extract(const BootArgs& A, int g,
 // ----------------------------------------------------------------------------
 // PF: the transforms request their twiddles a stage ahead (one wave per SIMD); without it
 // they read them just before use, which leaves registers for two waves per SIMD.
-// DIG8 (Goldilocks with 8 blind-rotation digits of 8 bits, the launcher's choice): the
-// digit loop reads byte digits from the gate's scratch (gadget_mac_dig8). A separate
-// instantiation, so that the register-held digit source of the other sets does not
-// weigh on its register allocation.
-template <class F, bool PF = true, bool DIG8 = false>
+// DIG (dig_bits of the set, the launcher's choice; 0 for the other sets): 8 or 5, the digit
+// loop reads byte digits from the gate's scratch (store_digits8 / store_digits5,
+// gadget_mac_dig8). A separate instantiation each, so that the register-held digit source
+// of the other sets does not weigh on their register allocation.
+template <class F, bool PF = true, int DIG = 0>
 __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typename DF<F>::W* lds,
                                                unsigned short* ms,
                                                const typename DF<F>::W* twl,
@@ -513,7 +575,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
   W* accA = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N : scratch;
   W* accB = accA + N;
   uint4* const dig = reinterpret_cast<uint4*>(scratch + 2 * N);
-  static_assert(!DIG8 || F::IS_GOLDILOCKS, "byte digits of 64-bit words");
+  static_assert(DIG == 0 || F::IS_GOLDILOCKS, "byte digits of 64-bit words");
 
   gate_modswitch<F>(A, g, lane, 64, ms);
   wave_sync();
@@ -549,8 +611,11 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
       rotate_sub<F, E>(comp ? accB : accA, r, traced, lane, dst);
       // c_neg_s_m for the a half, c_m for the b half (rlwe.rs:579-593).
       const W* ckey = key + (size_t)comp * A.br_len * 2 * N;
-      if constexpr (DIG8) {
-        store_digits8(dst, dig, lane);
+      if constexpr (DIG != 0) {
+        if constexpr (DIG == 8)
+          store_digits8(dst, dig, lane);
+        else
+          store_digits5(dst, dig, lane);
         gate_sync();
         gadget_mac_dig8<E, PF>(pa, pb, dig, A.br_len, ckey, lds, twl, lane);
       } else {
@@ -586,8 +651,9 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
 // each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then
 // key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
-// window (at most 4 digits) fits 32 bits; DIG8 (Goldilocks with 8 digits of 8 bits, 8 waves):
-// a wave's two byte digits go through the byte-digit transform (gadget_mac_bytes2).
+// window (at most 4 digits) fits 32 bits; DIG (dig_bits, 8 waves): a wave's digits go through
+// the byte-digit transform (two 8-bit digits: gadget_mac_bytes2; 5-bit digits:
+// gadget_mac_window_bytes).
 // The accumulator stays in LDS for the whole gate, so a round's last barrier waits for LDS
 // stores only and the next rotation reads LDS. With 8 waves each wave keeps the raw words
 // of the 16 positions it updates in registers for the whole gate, and LDS holds every
@@ -597,7 +663,7 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // (accA, accB: N words each) have their own LDS.
 // Untraced semantics only.
 // ----------------------------------------------------------------------------
-template <class F, int WAVES, bool DIG8 = false>
+template <class F, int WAVES, int DIG = 0>
 __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      typename DF<F>::W* lds, int ldsw,
                                                      unsigned short* ms,
@@ -607,7 +673,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      const typename DF<F>::W* twl,
                                                      const typename DF<F>::W* itwl) {
   static_assert(WAVES == 4 || WAVES == 8, "4 or 8 wavefronts per gate");
-  static_assert(!DIG8 || (F::IS_GOLDILOCKS && WAVES == 8), "two byte digits per wave");
+  static_assert(DIG == 0 || (F::IS_GOLDILOCKS && WAVES == 8), "byte digits with 8 waves");
   using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
@@ -663,13 +729,16 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
         rotate_sub<F, E>(ac, r, false, lane, dst);
       // digits j0..j1-1: shift the digit window by j0 digits, keys from row j0
       const W* kj = key + (size_t)j0 * 2 * N;
-      if constexpr (DIG8) {
+      if constexpr (DIG == 8) {
         gadget_mac_bytes2<PF>(pa, pb, dst, j0, kj, lds, twl, lane, wave >= 4);
       } else if constexpr (WAVES == 8) {
         u32 win[E];
 #pragma unroll
         for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
-        gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, twl, lane, wave >= 4);
+        if constexpr (DIG != 0)
+          gadget_mac_window_bytes<DIG, PF>(pa, pb, win, j1 - j0, kj, lds, twl, lane, wave >= 4);
+        else
+          gadget_mac<F, E, PF>(pa, pb, win, j1 - j0, A.br_bits, kj, lds, twl, lane, wave >= 4);
       } else {
         if (j0 > 0) {
 #pragma unroll
@@ -782,7 +851,7 @@ __device__ __forceinline__ void stage_twiddles(const void* tw, const void* itw, 
 // WPS: waves per SIMD the register allocation targets (1: up to 512 VGPR+AGPR per lane;
 // 2: at most 256, 3: at most 168, the excess spilled, so that the waves of a SIMD hide each
 // other's LDS and memory latency; 3 suits BabyBear's 32-bit words).
-template <class F, int GPW, int WPS, bool DIG8>
+template <class F, int GPW, int WPS, int DIG>
 __global__ __launch_bounds__(64 * GPW) __attribute__((amdgpu_waves_per_eu(WPS, WPS)))
 void bootstrap_kernel(BootArgs A) {
   using W = typename DF<F>::W;
@@ -796,11 +865,11 @@ void bootstrap_kernel(BootArgs A) {
   if (g >= A.count) return;  // after the only workgroup barrier
   // Twiddles a stage ahead with one wave per SIMD, and for BabyBear, whose 32-bit words
   // leave the registers for it (4096 gates at (4,2): 55.5 -> 54.9 ms).
-  bootstrap_gate<F, WPS == 1 || !F::IS_GOLDILOCKS, DIG8>(A, (int)g, lds[w], ms[w], twl, itwl);
+  bootstrap_gate<F, WPS == 1 || !F::IS_GOLDILOCKS, DIG>(A, (int)g, lds[w], ms[w], twl, itwl);
 }
 
 // Latency-mode launch: one gate per workgroup of WAVES wavefronts (bootstrap_gate_split).
-template <class F, int WAVES, bool DIG8>
+template <class F, int WAVES, int DIG>
 __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[WAVES][1024 + 64];
@@ -812,7 +881,7 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
   W* const aa = WAVES == 4 ? accl : part;
   W* const ab = WAVES == 4 ? accl + 1024 : part + 2 * 1024;
-  bootstrap_gate_split<F, WAVES, DIG8>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
+  bootstrap_gate_split<F, WAVES, DIG>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
                                        part, aa, ab, twl, itwl);
 }
 
@@ -831,10 +900,13 @@ __global__ __launch_bounds__(64) void mixed_kernel(const BootArgs* __restrict__ 
   const int g = (int)(b - A.first_block);
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64);
-    if (dig8_set(A))
-      bootstrap_gate<Gl, true, true>(A, g, lds, ms, twl, itwl);
+    const int db = dig_bits(A);
+    if (db == 8)
+      bootstrap_gate<Gl, true, 8>(A, g, lds, ms, twl, itwl);
+    else if (db == 5)
+      bootstrap_gate<Gl, true, 5>(A, g, lds, ms, twl, itwl);
     else
-      bootstrap_gate<Gl, true, false>(A, g, lds, ms, twl, itwl);
+      bootstrap_gate<Gl, true, 0>(A, g, lds, ms, twl, itwl);
   } else {
     u32* t = reinterpret_cast<u32*>(twl);
     u32* it = reinterpret_cast<u32*>(itwl);
@@ -867,13 +939,15 @@ void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * GPW);
     if (g >= A.count) return;
     if constexpr (CLS == 1)
-      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl, itwl);
+      bootstrap_gate<Gl, WPS == 1, 8>(A, (int)g, lds[w], ms[w], twl, itwl);
     else if constexpr (CLS == 2)
-      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl, itwl);
-    else if (dig8_set(A))
-      bootstrap_gate<Gl, WPS == 1, true>(A, (int)g, lds[w], ms[w], twl, itwl);
+      bootstrap_gate<Gl, WPS == 1, 5>(A, (int)g, lds[w], ms[w], twl, itwl);
+    else if (dig_bits(A) == 8)
+      bootstrap_gate<Gl, WPS == 1, 8>(A, (int)g, lds[w], ms[w], twl, itwl);
+    else if (dig_bits(A) == 5)
+      bootstrap_gate<Gl, WPS == 1, 5>(A, (int)g, lds[w], ms[w], twl, itwl);
     else
-      bootstrap_gate<Gl, WPS == 1, false>(A, (int)g, lds[w], ms[w], twl, itwl);
+      bootstrap_gate<Gl, WPS == 1, 0>(A, (int)g, lds[w], ms[w], twl, itwl);
   } else if constexpr (CLS == 0 || CLS == 3) {
     u32* t = reinterpret_cast<u32*>(twl);
     u32* it = reinterpret_cast<u32*>(itwl);
@@ -905,9 +979,12 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
     u64* const ab = WAVES == 4 ? aa + 1024 : aa + 2 * 1024;
-    if (WAVES == 8 && dig8_set(A))
-      bootstrap_gate_split<Gl, WAVES, WAVES == 8>(A, g, lds[w], 1024 + 64, ms, part, aa, ab, twl,
-                                                  itwl);
+    if (WAVES == 8 && dig_bits(A) == 8)
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 8 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
+                                                          ab, twl, itwl);
+    else if (WAVES == 8 && dig_bits(A) == 5)
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 5 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
+                                                          ab, twl, itwl);
     else
       bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, aa, ab, twl, itwl);
   } else {
@@ -1093,21 +1170,21 @@ hipError_t launch_lwe_not(const u64* in, u64* out, u32 count, u32 n1, u64 q, int
                      out, count, n1, q, is_goldilocks);
   return hipGetLastError();
 }
-template <class F, bool D8>
+template <class F, int DIG>
 static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStream_t s) {
   const dim3 g1(a.count), g2((a.count + 1) / 2), g4((a.count + 3) / 4);
   if (gpw == 1 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 1, 1, D8>), g1, dim3(64), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 1, 1, DIG>), g1, dim3(64), 0, s, a);
   else if (gpw == 2 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 1, D8>), g2, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 1, DIG>), g2, dim3(128), 0, s, a);
   else if (gpw == 4 && wps == 1)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 1, D8>), g4, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 1, DIG>), g4, dim3(256), 0, s, a);
   else if (gpw == 2 && wps == 2)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2, D8>), g2, dim3(128), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 2, 2, DIG>), g2, dim3(128), 0, s, a);
   else if (gpw == 4 && wps == 2)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2, D8>), g4, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 2, DIG>), g4, dim3(256), 0, s, a);
   else if (gpw == 4 && wps == 3)
-    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 3, D8>), g4, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((bootstrap_kernel<F, 4, 3, DIG>), g4, dim3(256), 0, s, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -1118,25 +1195,31 @@ hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int wave
   if (waves != 4 && waves != 8) return hipErrorInvalidValue;
   if (waves == 4) {
     if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4, false>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4, 0>), dim3(a.count), dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4, false>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4, 0>), dim3(a.count), dim3(256), 0, s, a);
   } else {
-    if (is_goldilocks && dig8_set(a))
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, true>), dim3(a.count), dim3(512), 0, s, a);
+    const int db = is_goldilocks ? dig_bits(a) : 0;
+    if (db == 8)
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 8>), dim3(a.count), dim3(512), 0, s, a);
+    else if (db == 5)
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 5>), dim3(a.count), dim3(512), 0, s, a);
     else if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, false>), dim3(a.count), dim3(512), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 0>), dim3(a.count), dim3(512), 0, s, a);
     else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8, false>), dim3(a.count), dim3(512), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8, 0>), dim3(a.count), dim3(512), 0, s, a);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s) {
-  if (!is_goldilocks) return launch_bootstrap_f<Bb, false>(a, gpw, wps, s);
-  return dig8_set(a) ? launch_bootstrap_f<Gl, true>(a, gpw, wps, s)
-                     : launch_bootstrap_f<Gl, false>(a, gpw, wps, s);
+  if (!is_goldilocks) return launch_bootstrap_f<Bb, 0>(a, gpw, wps, s);
+  switch (dig_bits(a)) {
+    case 8: return launch_bootstrap_f<Gl, 8>(a, gpw, wps, s);
+    case 5: return launch_bootstrap_f<Gl, 5>(a, gpw, wps, s);
+    default: return launch_bootstrap_f<Gl, 0>(a, gpw, wps, s);
+  }
 }
 
 hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates, int waves,

@@ -49,8 +49,9 @@ struct BootArgs {
 };
 
 // Per-gate scratch in BootArgs::acc, in words of the set's size: the accumulator RLWE (2N
-// words) then the rotated accumulator's 8-bit gadget digits, byte-transposed (N words).
-constexpr size_t kScratchWordsPerGate = 3 * 1024;
+// words) then the rotated accumulator's byte-sized gadget digits, byte-transposed (one
+// 16-byte word per lane and digit: 8 digits of 8 bits or 13 of 5 bits, at most 2N words).
+constexpr size_t kScratchWordsPerGate = 4 * 1024;
 
 // Batched-kernel variants: gpw gates (wavefronts) per workgroup and wps waves per SIMD
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2), (4,3)}.

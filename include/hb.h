@@ -162,7 +162,7 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_d
 
 /* No reference counterpart. For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): hb_gate_batch_dev
- * for gate kinds 0-6 with the kernels' per-gate scratch (accumulator and byte digits, 3N
+ * for gate kinds 0-6 with the kernels' per-gate scratch (accumulator and byte digits, 4N
  * words of the set's word size per gate) taken from a fixed device array of
  * HB_NOALLOC_SCRATCH_BYTES that the library reserves on its GPU for the life of the
  * process; the call allocates nothing. At most 1024 Goldilocks / Zama or 2048 BabyBear gates
@@ -170,7 +170,7 @@ int hb_gate_batch_dev(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_d
  * the last call's contents: calls whose kernels can overlap (other streams, other threads
  * without ordering) race and corrupt each other's results, so the caller serialises them.
  * Not timed by hb_last_kernel_ms. */
-#define HB_NOALLOC_SCRATCH_BYTES (24u << 20)
+#define HB_NOALLOC_SCRATCH_BYTES (32u << 20)
 int hb_gate_batch_dev_noalloc(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s_dev,
                               const uint64_t* c1s_dev, const uint64_t* c2s_dev,
                               uint64_t* outs_dev, void* stream);
