@@ -863,6 +863,34 @@ __device__ __forceinline__ int coop_a1(int p) { return p + (p >> 3); }
 __device__ __forceinline__ int coop_a2(int p) { return p + 4 * (p >> 4); }
 __device__ __forceinline__ int coop_a3(int p) { return p + 4 * (p >> 6); }
 
+// Gentleman-Sande pair of the N = 1024 inverse transform at the compile-time twiddle indices
+// IA, IB (psi^-bitrev(I)): Goldilocks shift forms where both have one (pow2_twiddle; every
+// index below 32 has), else the general product with the table words itwv[IA], itwv[IB].
+template <class F, int IA, int IB>
+__device__ __forceinline__ void gs2_const(typename DF<F>::W& xa, typename DF<F>::W& ya,
+                                          typename DF<F>::W& xb, typename DF<F>::W& yb,
+                                          const typename DF<F>::W* itwv) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (F::IS_GOLDILOCKS) {
+    constexpr Pow2Tw a = pow2_twiddle<10, true>(IA), b = pow2_twiddle<10, true>(IB);
+    if constexpr (a.ok && b.ok) {
+      gl_gs2_pow2<a.type, a.s1, a.s2, (bool)a.neg, b.type, b.s1, b.s2, (bool)b.neg>(xa, ya, xb,
+                                                                                   yb);
+      return;
+    }
+  }
+#endif
+  Bfly<F>::gs2_v(xa, ya, itwv[IA], xb, yb, itwv[IB]);
+}
+
+// Stages t = 64, 128 of quarter QW of the four-wave inverse transform.
+template <class F, int QW>
+__device__ __forceinline__ void inv_quarter_top(typename DF<F>::W (&x)[4],
+                                                const typename DF<F>::W* itwv) {
+  gs2_const<F, 8 + 2 * QW, 9 + 2 * QW>(x[0], x[1], x[2], x[3], itwv);
+  gs2_const<F, 4 + QW, 4 + QW>(x[0], x[2], x[1], x[3], itwv);
+}
+
 template <class F>
 __device__ __forceinline__ void ntt_inv_quarter(typename DF<F>::W (&x)[4], typename DF<F>::W* lds,
                                                 const typename DF<F>::W* itwv, int lane, int qw) {
@@ -903,9 +931,19 @@ __device__ __forceinline__ void ntt_inv_quarter(typename DF<F>::W (&x)[4], typen
   {  // t = 64, 128
 #pragma unroll
     for (int k = 0; k < 4; k++) x[k] = lds[coop_a3(lane + 64 * k)];
-    const W wa = itwv[8 + 2 * qw], wb = itwv[8 + 2 * qw + 1], w128 = itwv[4 + qw];
-    Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
-    Bfly<F>::gs2_v(x[0], x[2], w128, x[1], x[3], w128);
+    if constexpr (F::IS_GOLDILOCKS) {
+      // twiddle indices 8 + 2 qw, 9 + 2 qw and 4 + qw: shifts, per wave-uniform qw
+      switch (qw) {
+        case 0: inv_quarter_top<F, 0>(x, itwv); break;
+        case 1: inv_quarter_top<F, 1>(x, itwv); break;
+        case 2: inv_quarter_top<F, 2>(x, itwv); break;
+        default: inv_quarter_top<F, 3>(x, itwv); break;
+      }
+    } else {
+      const W wa = itwv[8 + 2 * qw], wb = itwv[8 + 2 * qw + 1], w128 = itwv[4 + qw];
+      Bfly<F>::gs2_v(x[0], x[1], wa, x[2], x[3], wb);
+      Bfly<F>::gs2_v(x[0], x[2], w128, x[1], x[3], w128);
+    }
 #pragma unroll
     for (int k = 0; k < 4; k++) lds[coop_a3(lane + 64 * k)] = x[k];
   }
@@ -922,9 +960,8 @@ __device__ __forceinline__ void ntt_inv_cross(typename DF<F>::W (&x)[4],
   const int a = coop_a3(64 * qw + lane);
 #pragma unroll
   for (int k = 0; k < 4; k++) x[k] = q0[(size_t)k * stride + a];
-  const W w2 = itwv[2], w3 = itwv[3], w1 = itwv[1];
-  Bfly<F>::gs2_v(x[0], x[1], w2, x[2], x[3], w3);
-  Bfly<F>::gs2_v(x[0], x[2], w1, x[1], x[3], w1);
+  gs2_const<F, 2, 3>(x[0], x[1], x[2], x[3], itwv);
+  gs2_const<F, 1, 1>(x[0], x[2], x[1], x[3], itwv);
 }
 
 }  // namespace hb
