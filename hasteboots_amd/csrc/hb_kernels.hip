@@ -707,8 +707,12 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   }
 
   const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
-  // this wave's digits: [j0, j1) of its component's gadget
-  const u32 j0 = (u32)q * A.br_len / Q, j1 = (u32)(q + 1) * A.br_len / Q;
+  // this wave's digits: [j0, j1) of its component's gadget, window qd of Q. With 8 waves
+  // the b component's windows run in reverse order, so that when the windows differ in
+  // length (Zama: 3, 3, 3, 4 digits) the SIMD running waves w and w + 4 does not get both
+  // long ones.
+  const int qd = (WAVES == 8 && comp) ? Q - 1 - q : q;
+  const u32 j0 = (u32)qd * A.br_len / Q, j1 = (u32)(qd + 1) * A.br_len / Q;
   // partial slots [4][a, b][e][lane]: with 4 waves wave w stores slot w; with 8 waves
   // wave w < 4 the a half of slot w and wave w + 4 its b half, each the sum of both waves'
   // partials
