@@ -45,6 +45,12 @@ struct Gl {
     if (s2 < s) s2 += EPS;
     return s2;
   }
+  // add for a canonical y: x + y < 2^64 + P, so the EPS of a carry cannot carry again and the
+  // second correction of add never applies (the same word, one compare fewer).
+  HB_HD static u64 add_cy(u64 x, u64 y) {
+    const u64 s = x + y;
+    return s + (s < x ? EPS : 0);
+  }
   // ops.rs:48-55, bit-exact.
   HB_HD static u64 sub(u64 x, u64 y) {
     u64 d = x - y;
@@ -136,6 +142,7 @@ struct Bb {
     const u32 s = a + b;
     return umin(s, s - P);
   }
+  HB_HD static u32 add_cy(u32 a, u32 b) { return add(a, b); }
   HB_HD static u32 sub(u32 a, u32 b) {
     const u32 d = a - b;
     return umin(d, d + P);

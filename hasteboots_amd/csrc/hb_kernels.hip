@@ -628,13 +628,13 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
-      outA[k] = F::add(accA[k], pa[e]);
+      outA[k] = F::add_cy(accA[k], pa[e]);
     }
     ntt_inv_wave<F, E, PF>(pb, lds, itwl, lane);
 #pragma unroll
     for (int e = 0; e < E; e++) {
       const int k = lane + 64 * e;
-      outB[k] = F::add(accB[k], pb[e]);
+      outB[k] = F::add_cy(accB[k], pb[e]);
     }
     gate_sync();
   }
@@ -791,7 +791,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int pos = 64 * qw + lane + 256 * k;
-        raw[k] = F::add(raw[k], x[k]);
+        raw[k] = F::add_cy(raw[k], x[k]);
         acc[pos] = F::canon(raw[k]);
         acc[N + pos] = F::canon(F::neg_raw(raw[k]));
       }
@@ -819,8 +819,8 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int pos = 64 * qw + lane + 256 * k;
-        accA[pos] = F::add(accA[pos], xa[k]);
-        accB[pos] = F::add(accB[pos], xb[k]);
+        accA[pos] = F::add_cy(accA[pos], xa[k]);
+        accB[pos] = F::add_cy(accB[pos], xb[k]);
       }
     }
     __syncthreads();

@@ -101,3 +101,52 @@ def test_babybear_ops_closed_form(orc):
     assert (sub.astype(object) == (xi - yi) % BP).all()
     assert (neg.astype(object) == (-xi) % BP).all()
     assert (mul.astype(object) == (xi * yi) % BP).all()
+
+
+def test_device_add_and_rotation_shortcuts(orc):
+    """The kernels' rewrites of the reference's raw-word steps give the same words.
+
+    add_cy (hb_field.hpp): for a canonical addend the carry's EPS cannot carry again, so one
+    correction gives add's word (ops.rs:27-34). rotate_sub (Goldilocks): neg_raw(v) is
+    replaced by P - v mod 2^64 (they differ only at v = 0, where P = 0 mod P) and the
+    difference is one raw sub (ops.rs:48-55) then a canonicalisation; the reference
+    (rlwe.rs:445-481) negates the raw word, so for v > P the result is -v + EPS, not -v."""
+    rng = np.random.default_rng(5)
+    rand = [int(v) for v in rng.integers(0, 2**63, 200, dtype=np.uint64) * 2 + 1]
+    words = EDGES + rand
+    canon = [w for w in words if w < GP]
+    xs = np.array([x for x in words for _ in canon], dtype=np.uint64)
+    ys = np.array([y for _ in words for y in canon], dtype=np.uint64)
+    add = orc.gl_op(0, xs, ys)
+    for x, y, r in zip(xs.tolist(), ys.tolist(), add.tolist()):
+        s = (x + y) % M64
+        assert r == (s + (EPS if x + y >= M64 else 0)) % M64
+    # rotation: dst = canon(sub(P - v, a)) against (neg_raw(v) - a) mod P and
+    # canon(sub(v, a)) against (v - a) mod P, for raw v and a
+    vs = np.array([v for v in words for _ in words], dtype=np.uint64)
+    As = np.array([a for _ in words for a in words], dtype=np.uint64)
+    negv = np.array([(GP - v) % M64 for v in vs.tolist()], dtype=np.uint64)
+    d_neg = orc.gl_op(1, negv, As)
+    d_pos = orc.gl_op(1, vs, As)
+    for v, a, dn, dp in zip(vs.tolist(), As.tolist(), d_neg.tolist(), d_pos.tolist()):
+        assert dn % GP == (ref_neg(v) - a) % GP
+        assert dp % GP == (v - a) % GP
+
+
+def test_rotation_index_of_canon_and_negated_halves():
+    """rotate_sub_cn reads word (k - r) mod 2N of [C | M] (C = canon(acc), M =
+    canon(neg_raw(acc))): the same source word and sign as the reference's negacyclic
+    rotation (rlwe.rs:445-481: for r <= N the words k < r wrap with a sign flip, for r > N
+    the rotation by r - N flips the words k >= r - N)."""
+    for n in (8, 64, 1024):
+        for r in range(1, 2 * n):
+            for k in range(n):
+                if r <= n:
+                    neg, src = k < r, (k - r + n if k < r else k - r)
+                else:
+                    rr = r - n
+                    neg, src = k >= rr, (k - rr if k >= rr else k - rr + n)
+                t = (k - r) & (2 * n - 1)
+                assert (t >= n) == neg and t % n == src, (n, r, k)
+            if n == 1024 and r > 64:
+                break
