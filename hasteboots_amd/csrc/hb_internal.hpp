@@ -67,19 +67,18 @@ struct Engine {
   uint32_t simd_slots = 1024;
   // Per parameter set, batches of at most split_max[set] untraced gates run in latency
   // mode (split_waves wavefronts per gate, one workgroup per gate). Defaults
-  // (default_split_max): three gates per CU for the Goldilocks sets and one per SIMD for
-  // BabyBear. Measured on MI355X (256 CUs), Goldilocks: 256 gates 17.9 ms, 512 35.8 ms,
-  // 768 53.8 ms in latency mode against 67-71 ms with one wavefront per gate, which ties at
-  // 1024 gates (72.0 against 71.6 ms); Zama loses there (104 against 113 ms), BabyBear still
-  // wins (16.4 against 19.5 ms).
-  uint32_t split_max[3] = {1024, 768, 768};
+  // (default_split_max): seven gates per CU for Goldilocks and BabyBear, six for Zama, the
+  // largest batches where latency mode still wins. Measured on MI355X (256 CUs), kernel ms
+  // in latency mode / the default path above the cutoff: Goldilocks 1792 gates 98.5 /
+  // 103.8, 2048 112.6 / 104.1; Zama 1536 138.5 / 159.0, 1792 161.7 / 160.4; BabyBear 1792
+  // 27.8 / 28.5, 2048 31.5 / 28.5.
+  uint32_t split_max[3] = {1792, 1792, 1536};
 };
 
 // Built-in latency-mode cutoff of a parameter set on a device with `simd_slots` SIMDs.
 inline uint32_t default_split_max(int set, uint32_t simd_slots) {
-  const uint32_t three_per_cu = 3 * (simd_slots / 4);
-  return set == SET_BABYBEAR ? (simd_slots > three_per_cu ? simd_slots : three_per_cu)
-                             : three_per_cu;
+  const uint32_t cus = simd_slots / 4;
+  return (set == SET_ZAMA ? 6 : 7) * cus;
 }
 
 // Latency-mode cutoff of a launch mixing the parameter sets in the bit mask `sets`: the

@@ -213,10 +213,10 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * graphs) of at most max_gates gates run 8 wavefronts per gate (4 for BabyBear launches of
  * more than one gate per compute unit), each wave taking a share of the digit transforms
  * and of the inverse transforms, which cuts a gate's latency about 4x at lower peak
- * throughput. Each parameter set has its own cutoff. Defaults: three gates per compute unit
- * (768 on MI355X, where latency mode stays faster than one wavefront per gate) for the
- * Goldilocks sets, and one per SIMD (1024) for BabyBear, whose latency mode stays ahead
- * that far. hb_set_latency_mode sets the cutoff of every set, hb_set_latency_mode_for of
+ * throughput. Each parameter set has its own cutoff. Defaults: seven gates per compute
+ * unit (1792 on MI355X) for Goldilocks and BabyBear and six (1536) for Zama, the largest
+ * batches at which latency mode still beats the batched kernels on MI355X.
+ * hb_set_latency_mode sets the cutoff of every set, hb_set_latency_mode_for of
  * one; 0 disables latency mode, HB_LATENCY_DEFAULT restores the built-in cutoff.
  * hb_latency_mode reports the Goldilocks cutoff, hb_latency_mode_for any set's. */
 #define HB_LATENCY_DEFAULT 0xFFFFFFFFu

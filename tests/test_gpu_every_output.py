@@ -115,7 +115,8 @@ def test_latency_cutoffs_are_per_set_and_restorable(hb):
         assert hb.latency_modes() == (modes[0], modes[1], 5)
         hb.set_latency_mode(hb.LATENCY_DEFAULT)
         defaults = hb.latency_modes()
-        assert defaults[0] >= defaults[1] == defaults[2] > 0
+        # BabyBear, Goldilocks, Zama: Zama's heavier gates leave latency mode first
+        assert defaults[0] == defaults[1] >= defaults[2] > 0
     finally:
         hb.restore_latency_modes(modes)
     assert hb.latency_modes() == modes

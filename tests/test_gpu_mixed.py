@@ -142,8 +142,18 @@ def test_mixed_throughput_shape_every_output(hb, orc, keyset):
 @pytest.mark.parametrize("set_id", [0, 1, 2], ids=["babybear", "goldilocks", "zama"])
 def test_single_set_throughput_shape_matches_batched_kernel(hb, orc, keyset, set_id):
     """A mixed launch of one parameter set above one gate per SIMD runs that set's own
-    kernel body (one class); two clients' parts of 550 gates each: every output equals the
-    dedicated batched launch's, and a sample the oracle's."""
+    kernel body (one class); two clients' parts of 550 gates each, latency mode off so that
+    the launch takes the throughput shape: every output equals the dedicated launch's, and
+    a sample the oracle's."""
+    modes = hb.latency_modes()
+    hb.set_latency_mode(0)
+    try:
+        _single_set_throughput_shape(hb, orc, keyset, set_id)
+    finally:
+        hb.restore_latency_modes(modes)
+
+
+def _single_set_throughput_shape(hb, orc, keyset, set_id):
     rng = stream(44 + set_id, "single-class")
     parts, data = [], []
     for seed in (0, 1):
