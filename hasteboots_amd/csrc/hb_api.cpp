@@ -24,7 +24,7 @@ using namespace hb;
 namespace {
 thread_local std::string g_err;
 
-// Per-gate scratch of hb_gate_batch_dev_noalloc (accumulator and byte digits, 3N words of
+// Per-gate scratch of hb_gate_batch_dev_noalloc (accumulator and byte digits, 4N words of
 // the set's word size per gate), reserved on every GPU the library runs on for the life of
 // the process. Host code uses the address hipGetSymbolAddress returns (Engine::
 // noalloc_scratch), never this name.
@@ -410,6 +410,39 @@ BootArgs hb::make_args(const hb_ek* ek, int gate, size_t count, const uint64_t* 
 
 namespace {
 
+// Launches the untraced batch `a` of parameter set p in the shape batch_variant picks. A
+// batch of the tuned batched variant whose last round of resident gates (waves per SIMD x
+// SIMDs) would be partly filled runs its whole rounds batched and the rest, when latency
+// mode takes that many, in latency mode after them: Goldilocks 2560 gates 157 ms in one
+// batched launch against 104 + 28 ms. The parts write disjoint rows; the latency-mode
+// kernel keeps its accumulators in LDS.
+hipError_t launch_shaped(const Engine& E, const ParamSet& p, const BootArgs& a, hipStream_t s) {
+  const size_t count = a.count;
+  int gpw, wps;
+  batch_variant(E, p.set, count, &gpw, &wps);
+  auto split = [&](const BootArgs& b) {
+    return launch_bootstrap_split(
+        b, p.is_goldilocks, split_waves(E, p.is_goldilocks, b.count, split8_fits(p.br_len, p.br_bits)),
+        s);
+  };
+  if (gpw == 0) return split(a);
+  const size_t round = (size_t)wps * E.simd_slots, rest = count % round;
+  if (gpw > 1 && wps <= 2 && !a.trace && !a.idx && count > round && rest > 0 &&
+      rest <= E.split_max[p.set]) {
+    BootArgs head = a, tail = a;
+    head.count = (uint32_t)(count - rest);
+    tail.count = (uint32_t)rest;
+    const size_t off = (size_t)head.count * (p.n + 1);
+    tail.c0 = a.c0 + off;
+    if (a.c1) tail.c1 = a.c1 + off;
+    if (a.c2) tail.c2 = a.c2 + off;
+    tail.out = a.out + off;
+    hipError_t e = launch_bootstrap(head, p.is_goldilocks, gpw, wps, s);
+    return e != hipSuccess ? e : split(tail);
+  }
+  return launch_bootstrap(a, p.is_goldilocks, gpw, wps, s);
+}
+
 int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
                  const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {
   const ParamSet& p = ek->p;
@@ -421,14 +454,7 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   if (rc) return rc;
   rc = timer_start(s);
   if (rc) return rc;
-  int gpw, wps;
-  batch_variant(E, p.set, count, &gpw, &wps);
-  if (gpw == 0)
-    HIP_TRY(launch_bootstrap_split(
-        a, p.is_goldilocks,
-        split_waves(E, p.is_goldilocks, count, split8_fits(p.br_len, p.br_bits)), s));
-  else
-    HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
+  HIP_TRY(launch_shaped(E, p, a, s));
   rc = timer_stop(s);
   if (rc) return rc;
   return acc_release(ek, s);
@@ -681,14 +707,7 @@ int hb_gate_batch_dev_noalloc(hb_ek_t* ek, int gate, size_t count, const uint64_
   // The kernels write every scratch word they read (the accumulator is initialised from
   // the LUT, the digits from the rotated accumulator), so nothing is cleared first.
   BootArgs a = make_args(ek, gate, count, c0, c1, c2, nullptr, outs, E.noalloc_scratch);
-  int gpw, wps;
-  batch_variant(E, p.set, count, &gpw, &wps);
-  if (gpw == 0)
-    HIP_TRY(launch_bootstrap_split(
-        a, p.is_goldilocks,
-        split_waves(E, p.is_goldilocks, count, split8_fits(p.br_len, p.br_bits)), s));
-  else
-    HIP_TRY(launch_bootstrap(a, p.is_goldilocks, gpw, wps, s));
+  HIP_TRY(launch_shaped(E, p, a, s));
   return HB_OK;
 }
 
@@ -1130,12 +1149,7 @@ int hb_batch_graph_create(hb_ek_t* ek, int gate, size_t count, const uint64_t* c
   }
   e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int gpw, wps;
-    batch_variant(E, ek->p.set, count, &gpw, &wps);
-    const int waves = split_waves(E, ek->p.is_goldilocks, count,
-                                  split8_fits(ek->p.br_len, ek->p.br_bits));
-    hipError_t le = gpw == 0 ? launch_bootstrap_split(a, ek->p.is_goldilocks, waves, cs)
-                             : launch_bootstrap(a, ek->p.is_goldilocks, gpw, wps, cs);
+    hipError_t le = launch_shaped(E, ek->p, a, cs);
     e = hipStreamEndCapture(cs, &g->graph);
     if (le != hipSuccess) e = le;
   }

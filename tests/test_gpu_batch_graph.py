@@ -41,16 +41,18 @@ def _inputs(hb, sk, seed, count):
 def test_variants_match_oracle(hb, orc, gold, variant):
     P, sk, bsk, ksk, ev = gold
     say(f"variant gates/workgroup, waves/SIMD = {variant}")
-    # more gates than SIMDs so the pinned variant is the one launched; odd, so the last
-    # workgroup of the 2- and 4-gate variants is partly empty
+    # more gates than SIMDs and latency mode off, so the pinned variant is the one
+    # launched; odd, so the last workgroup of the 2- and 4-gate variants is partly empty
     count = 1024 + 9
     b0, b1, c0, c1 = _inputs(hb, sk, 50, count)  # same batch for every variant
-    old = hb.tuned_config(hb.SET_GOLDILOCKS)
+    old, modes = hb.tuned_config(hb.SET_GOLDILOCKS), hb.latency_modes()
     hb.set_tuned_config(hb.SET_GOLDILOCKS, *variant)
+    hb.set_latency_mode(0)
     try:
         out = ev.gate_batch("nand", c0, c1)
     finally:
         hb.set_tuned_config(hb.SET_GOLDILOCKS, old[0], old[1])
+        hb.restore_latency_modes(modes)
     sample = np.r_[0:6, count - 6:count]  # both ends of the grid, against the oracle
     ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=12)
     assert np.array_equal(out[sample], ref)
@@ -164,12 +166,14 @@ def test_babybear_variants_match_oracle(hb, orc, keyset, variant):
     ev = _EVALUATORS[0]
     count = 1024 + 9
     b0, b1, c0, c1 = _inputs_for(hb, sk, 0, 93, count)
-    old = hb.tuned_config(hb.SET_BABYBEAR)
+    old, modes = hb.tuned_config(hb.SET_BABYBEAR), hb.latency_modes()
     hb.set_tuned_config(hb.SET_BABYBEAR, *variant)
+    hb.set_latency_mode(0)
     try:
         out = ev.gate_batch("nand", c0, c1)
     finally:
         hb.set_tuned_config(hb.SET_BABYBEAR, old[0], old[1])
+        hb.restore_latency_modes(modes)
     sample = np.r_[0:4, count - 4:count]
     ref = orc.OracleKeys(0, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=8)
     assert np.array_equal(out[sample], ref)
@@ -194,3 +198,27 @@ def test_latency_mode_matches_wave_per_gate(hb, orc, gold):
     assert np.array_equal(split, single)
     ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[:8], c1[:8], threads=8)
     assert np.array_equal(split[:8], ref)
+
+
+def test_partly_filled_round_runs_its_rest_in_latency_mode(hb, orc, gold):
+    """2048 + 300 Goldilocks gates: with the (4, 2) variant the whole round of 2048 resident
+    gates runs batched and the 300 left in latency mode after it (two launches); every
+    output equals the one batched launch's (latency mode off), a sample the oracle's."""
+    P, sk, bsk, ksk, ev = gold
+    count = 2048 + 300
+    b0, b1, c0, c1 = _inputs(hb, sk, 51, count)
+    old, modes = hb.tuned_config(hb.SET_GOLDILOCKS), hb.latency_modes()
+    hb.set_tuned_config(hb.SET_GOLDILOCKS, 4, 2)
+    try:
+        say(f"{count} gates, default launch shape")
+        shaped = ev.gate_batch("nand", c0, c1)
+        hb.set_latency_mode(0)
+        say(f"{count} gates, one batched launch")
+        whole = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_tuned_config(hb.SET_GOLDILOCKS, old[0], old[1])
+        hb.restore_latency_modes(modes)
+    assert np.array_equal(shaped, whole)
+    sample = np.r_[2040:2056, count - 4:count]  # both sides of the split
+    ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=12)
+    assert np.array_equal(shaped[sample], ref)

@@ -239,8 +239,9 @@ def test_modulus_switch_rounding_boundaries(hb, orc, keyset, set_id):
 
 
 def test_zama_batch_through_tuned_variant(hb, orc, keyset):
-    """Zama-Goldilocks (13 five-bit digits, digits held in registers) through the (4, 2)
-    batched variant: more gates than SIMDs, the last workgroup partly empty."""
+    """Zama-Goldilocks (13 five-bit digits, stored as byte digits) through the (4, 2)
+    batched variant, latency mode off: more gates than SIMDs, the last workgroup partly
+    empty."""
     P, sk, bsk, ksk = keyset(2)
     count = 1024 + 9
     say(f"zama batch {count} through (4, 2)")
@@ -249,12 +250,14 @@ def test_zama_batch_through_tuned_variant(hb, orc, keyset):
     enc = hb.Encryptor(sk)
     c0, c1 = enc.encrypt_many(b0), enc.encrypt_many(b1)
     ev = hb.Evaluator(keys=(bsk, ksk), params=P)
-    old = hb.tuned_config(hb.SET_ZAMA)
+    old, modes = hb.tuned_config(hb.SET_ZAMA), hb.latency_modes()
     hb.set_tuned_config(hb.SET_ZAMA, 4, 2)
+    hb.set_latency_mode(0)
     try:
         out = ev.gate_batch("nand", c0, c1)
     finally:
         hb.set_tuned_config(hb.SET_ZAMA, old[0], old[1])
+        hb.restore_latency_modes(modes)
     sample = np.r_[0:2, count - 2:count]
     ref = orc.OracleKeys(2, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=4)
     assert np.array_equal(out[sample], ref)
