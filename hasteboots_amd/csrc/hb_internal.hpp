@@ -125,7 +125,20 @@ inline int mixed_split_waves(bool narrow) { return narrow ? 8 : 4; }
 // wavefront per gate up to one gate per SIMD, the (4, 2) batched shape above that.
 // Traced gates have no latency-mode kernel.
 enum class MixedKernel { split, one_wave, batch };
+// Whether every Goldilocks set in `sets` has a byte-digit blind-rotation gadget (8 digits
+// of 8 bits or 13 of 5, as every set of param_set has): the mixed latency-mode and
+// throughput kernels hold only those Goldilocks bodies; any other gadget goes to the
+// one-wave kernel.
+inline bool byte_digit_sets(int sets) {
+  for (int set = 0; set < SET_COUNT; set++) {
+    ParamSet p;
+    if (!(sets & (1 << set)) || !param_set(set, &p) || !p.is_goldilocks) continue;
+    if (!((p.br_bits == 8 && p.br_len == 8) || (p.br_bits == 5 && p.br_len == 13))) return false;
+  }
+  return true;
+}
 inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced) {
+  if (!byte_digit_sets(sets)) return MixedKernel::one_wave;
   const uint32_t cutoff = mixed_split_cutoff(E, sets);
   const bool mixes = (sets & (sets - 1)) != 0;
   if (!traced && (gates <= cutoff || (cutoff > 0 && mixes && gates <= E.simd_slots)))

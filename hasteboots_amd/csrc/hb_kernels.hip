@@ -946,12 +946,10 @@ void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
       bootstrap_gate<Gl, WPS == 1, 8>(A, (int)g, lds[w], ms[w], twl, itwl);
     else if constexpr (CLS == 2)
       bootstrap_gate<Gl, WPS == 1, 5>(A, (int)g, lds[w], ms[w], twl, itwl);
-    else if (dig_bits(A) == 8)
+    else if (dig_bits(A) == 8)  // byte-digit gadgets only, as in mixed_split_kernel
       bootstrap_gate<Gl, WPS == 1, 8>(A, (int)g, lds[w], ms[w], twl, itwl);
-    else if (dig_bits(A) == 5)
-      bootstrap_gate<Gl, WPS == 1, 5>(A, (int)g, lds[w], ms[w], twl, itwl);
     else
-      bootstrap_gate<Gl, WPS == 1, 0>(A, (int)g, lds[w], ms[w], twl, itwl);
+      bootstrap_gate<Gl, WPS == 1, 5>(A, (int)g, lds[w], ms[w], twl, itwl);
   } else if constexpr (CLS == 0 || CLS == 3) {
     u32* t = reinterpret_cast<u32*>(twl);
     u32* it = reinterpret_cast<u32*>(itwl);
@@ -983,10 +981,13 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
   if (A.is_goldilocks) {
     stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
     u64* const ab = WAVES == 4 ? aa + 1024 : aa + 2 * 1024;
+    // 8 waves: the Goldilocks parts have byte-digit gadgets (mixed_kernel_for sends any
+    // other to the one-wave kernel), so the kernel holds two Goldilocks bodies, not three,
+    // and allocates registers like the dedicated kernels
     if (WAVES == 8 && dig_bits(A) == 8)
       bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 8 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
                                                           ab, twl, itwl);
-    else if (WAVES == 8 && dig_bits(A) == 5)
+    else if (WAVES == 8)
       bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 5 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
                                                           ab, twl, itwl);
     else
