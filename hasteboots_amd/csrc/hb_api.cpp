@@ -906,8 +906,12 @@ int traced_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, const
   a.trace = tr;
   if (acc_acquire(ek, E.stream) || timer_start(E.stream))
     return cleanup(HB_ERR_DEVICE);
-  if (launch_bootstrap(a, p.is_goldilocks, 1, 1, E.stream) != hipSuccess)
-    return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
+  // latency mode (8 waves, the accumulator history written from LDS) up to the set's
+  // cutoff, else one wavefront per gate with the history as its accumulator
+  const bool split = count <= E.split_max[p.set] && split8_fits(p.br_len, p.br_bits);
+  const hipError_t le = split ? launch_bootstrap_split(a, p.is_goldilocks, 8, E.stream)
+                              : launch_bootstrap(a, p.is_goldilocks, 1, 1, E.stream);
+  if (le != hipSuccess) return cleanup(fail(HB_ERR_DEVICE, "traced launch failed"));
   if (timer_stop(E.stream) || acc_release(ek, E.stream)) return cleanup(HB_ERR_DEVICE);
   std::vector<uint8_t> host(words * wb);
   if (hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream) != hipSuccess ||

@@ -123,7 +123,7 @@ inline int mixed_split_waves(bool narrow) { return narrow ? 8 : 4; }
 // latency mode up to the sets' smallest cutoff (or up to one gate per SIMD when the launch
 // mixes sets: heaviest parts first, the tail workgroups hold the lightest gates), one
 // wavefront per gate up to one gate per SIMD, the (4, 2) batched shape above that.
-// Traced gates have no latency-mode kernel.
+// Traced gates take latency mode only where it runs 8 waves (`narrow`).
 enum class MixedKernel { split, one_wave, batch };
 // Whether every Goldilocks set in `sets` has a byte-digit blind-rotation gadget (8 digits
 // of 8 bits or 13 of 5, as every set of param_set has): the mixed latency-mode and
@@ -137,11 +137,13 @@ inline bool byte_digit_sets(int sets) {
   }
   return true;
 }
-inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced) {
+inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced,
+                                    bool narrow = true) {
   if (!byte_digit_sets(sets)) return MixedKernel::one_wave;
   const uint32_t cutoff = mixed_split_cutoff(E, sets);
   const bool mixes = (sets & (sets - 1)) != 0;
-  if (!traced && (gates <= cutoff || (cutoff > 0 && mixes && gates <= E.simd_slots)))
+  // traced gates run the latency-mode kernel with 8 waves only
+  if ((!traced || narrow) && (gates <= cutoff || (cutoff > 0 && mixes && gates <= E.simd_slots)))
     return MixedKernel::split;
   return gates <= E.simd_slots ? MixedKernel::one_wave : MixedKernel::batch;
 }
@@ -156,10 +158,11 @@ inline int mixed_class(int sets) {
   return 0;
 }
 inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, int nparts,
-                                      uint32_t blocks, bool narrow, int sets, hipStream_t s) {
+                                      uint32_t blocks, bool narrow, int sets, hipStream_t s,
+                                      bool traced = false) {
   switch (k) {
     case MixedKernel::split:
-      return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), s);
+      return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), traced, s);
     case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
     default: return launch_mixed_batch(parts_dev, nparts, blocks, mixed_class(sets), s);
   }

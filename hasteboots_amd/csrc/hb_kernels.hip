@@ -646,24 +646,25 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
 // c = w / Q of the rotated accumulator (Q = WAVES / 2 waves per component) and the q = w % Q
 // share of its gadget digits, [q l / Q, (q + 1) l / Q), accumulating a partial NTT-domain
 // product. The partials meet in LDS (with 8 waves, waves w and w + 4 first add theirs
-// pairwise, a half each). Then every wave finishes quarters of the result (sum of the partials, inverse
-// NTT, accumulator add) with ntt_inv_quarter / ntt_inv_cross, exchanging through the
-// waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of the a half
-// each and waves 4-7 of the b half; with 4, wave w quarter w of both halves. Wave 0 then
-// key-switches. With 8 waves each SIMD holds two waves of the gate, and a wave's digit
-// window (at most 4 digits) fits 32 bits; DIG (dig_bits, 8 waves): a wave's digits go through
-// the byte-digit transform (two 8-bit digits: gadget_mac_bytes2; 5-bit digits:
-// gadget_mac_window_bytes).
+// pairwise, a half each). Then every wave finishes quarters of the result (sum of the
+// partials, inverse NTT, accumulator add) with ntt_inv_quarter / ntt_inv_cross, exchanging
+// through the waves' LDS buffers, `ldsw` words apart: with 8 waves, waves 0-3 a quarter of
+// the a half each and waves 4-7 of the b half; with 4, wave w quarter w of both halves.
+// Wave 0 then key-switches. With 8 waves each SIMD holds two waves of the gate, and a
+// wave's digit window (at most 4 digits) fits 32 bits; DIG (dig_bits, 8 waves): a wave's
+// digits go through the byte-digit transform (two 8-bit digits: gadget_mac_bytes2; 5-bit
+// digits: gadget_mac_window_bytes).
 // The accumulator stays in LDS for the whole gate, so a round's last barrier waits for LDS
 // stores only and the next rotation reads LDS. With 8 waves each wave keeps the raw words
 // of the 16 positions it updates in registers for the whole gate, and LDS holds every
 // component as rotate_sub_cn's [C | M] (accA = part, slot 0; accB = part + 2N, slot 1;
 // the slots are written only after the first barrier of a round, the partials of waves w
 // and w + 4 meeting through their exchange buffers first). With 4 waves the raw words
-// (accA, accB: N words each) have their own LDS.
-// Untraced semantics only.
+// (accA, accB: N words each) have their own LDS. TR: instantiated for traced gates
+// (A.trace set; 8 waves only, which the launchers keep to), so the untraced kernels carry
+// none of it.
 // ----------------------------------------------------------------------------
-template <class F, int WAVES, int DIG = 0>
+template <class F, int WAVES, int DIG = 0, bool TR = false>
 __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
                                                      typename DF<F>::W* lds, int ldsw,
                                                      unsigned short* ms,
@@ -692,7 +693,13 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   gate_lut_init<F>(A, tid, 64 * WAVES, ms, accA, accB);
   __syncthreads();
   // 8 waves: the raw words of positions 64 q + lane + 256 k of component `comp`, which this
-  // wave updates at the end of every round, and the [C | M] form of the component
+  // wave updates at the end of every round, and the [C | M] form of the component. Traced
+  // (the vfhe path, 8 waves only): M negates the canonical word (rotate_sub's traced
+  // semantics), and every wave writes its raw words to the gate's history slot by slot
+  // (slot 0 the initial accumulator, slot i + 1 the one after round i, copied for a
+  // skipped round), the layout of bootstrap_gate's traced accumulator.
+  const bool traced = TR && WAVES == 8 && A.trace != nullptr;
+  W* const hist = traced ? static_cast<W*>(A.trace) + (size_t)g * (n + 1) * 2 * N : nullptr;
   W raw[4];
   if constexpr (WAVES == 8) {
     W* ac = comp ? accB : accA;
@@ -701,7 +708,8 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
       const int pos = 64 * q + lane + 256 * k;
       raw[k] = ac[pos];
       ac[pos] = F::canon(raw[k]);
-      ac[N + pos] = F::canon(F::neg_raw(raw[k]));
+      ac[N + pos] = F::canon(F::neg_raw(traced ? F::canon(raw[k]) : raw[k]));
+      if (traced) hist[comp * N + pos] = raw[k];
     }
     __syncthreads();
   }
@@ -719,7 +727,16 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   W* mine = part + (size_t)(wave & 3) * 2 * N;
   for (u32 i = 0; i < n; i++) {
     const int r = ms[i];
-    if (r == 0) continue;  // uniform across the workgroup
+    if (r == 0) {  // uniform across the workgroup
+      if constexpr (WAVES == 8) {
+        if (traced) {
+#pragma unroll
+          for (int k = 0; k < 4; k++)
+            hist[(size_t)(i + 1) * 2 * N + comp * N + 64 * q + lane + 256 * k] = raw[k];
+        }
+      }
+      continue;
+    }
     const W* key = bsk + (size_t)i * rgsw_words + (size_t)comp * A.br_len * 2 * N;
     W pa[E], pb[E];
 #pragma unroll
@@ -793,7 +810,8 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
         const int pos = 64 * qw + lane + 256 * k;
         raw[k] = F::add_cy(raw[k], x[k]);
         acc[pos] = F::canon(raw[k]);
-        acc[N + pos] = F::canon(F::neg_raw(raw[k]));
+        acc[N + pos] = F::canon(F::neg_raw(traced ? F::canon(raw[k]) : raw[k]));
+        if (traced) hist[(size_t)(i + 1) * 2 * N + c * N + pos] = raw[k];
       }
     } else {
       // wave w finishes quarter w of both halves: the a quarter in words [0, 320) of its
@@ -873,7 +891,7 @@ void bootstrap_kernel(BootArgs A) {
 }
 
 // Latency-mode launch: one gate per workgroup of WAVES wavefronts (bootstrap_gate_split).
-template <class F, int WAVES, int DIG>
+template <class F, int WAVES, int DIG, bool TR>
 __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A) {
   using W = typename DF<F>::W;
   __shared__ W lds[WAVES][1024 + 64];
@@ -885,7 +903,7 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * WAVES);
   W* const aa = WAVES == 4 ? accl : part;
   W* const ab = WAVES == 4 ? accl + 1024 : part + 2 * 1024;
-  bootstrap_gate_split<F, WAVES, DIG>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
+  bootstrap_gate_split<F, WAVES, DIG, TR>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
                                        part, aa, ab, twl, itwl);
 }
 
@@ -960,8 +978,8 @@ void mixed_batch_kernel(const BootArgs* __restrict__ parts, int nparts) {
 }
 
 // Latency-mode mixed launch: one gate per workgroup of WAVES wavefronts, parts as in
-// mixed_kernel (untraced parts only).
-template <int WAVES>
+// mixed_kernel; TR (8 waves): some parts are traced.
+template <int WAVES, bool TR>
 __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
     const BootArgs* __restrict__ parts, int nparts) {
   __shared__ u64 lds[WAVES][1024 + 64];
@@ -985,11 +1003,11 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
     // other to the one-wave kernel), so the kernel holds two Goldilocks bodies, not three,
     // and allocates registers like the dedicated kernels
     if (WAVES == 8 && dig_bits(A) == 8)
-      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 8 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
-                                                          ab, twl, itwl);
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 8 : 0, TR>(A, g, lds[w], 1024 + 64, ms, part,
+                                                              aa, ab, twl, itwl);
     else if (WAVES == 8)
-      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 5 : 0>(A, g, lds[w], 1024 + 64, ms, part, aa,
-                                                          ab, twl, itwl);
+      bootstrap_gate_split<Gl, WAVES, WAVES == 8 ? 5 : 0, TR>(A, g, lds[w], 1024 + 64, ms, part,
+                                                              aa, ab, twl, itwl);
     else
       bootstrap_gate_split<Gl, WAVES>(A, g, lds[w], 1024 + 64, ms, part, aa, ab, twl, itwl);
   } else {
@@ -998,8 +1016,8 @@ __global__ __launch_bounds__(64 * WAVES) void mixed_split_kernel(
     stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64 * WAVES);
     u32* const ba = reinterpret_cast<u32*>(aa);
     u32* const bb = WAVES == 4 ? ba + 1024 : ba + 2 * 1024;
-    bootstrap_gate_split<Bb, WAVES>(A, g, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64), ms,
-                                    reinterpret_cast<u32*>(part), ba, bb, t, it);
+    bootstrap_gate_split<Bb, WAVES, 0, TR>(A, g, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64),
+                                           ms, reinterpret_cast<u32*>(part), ba, bb, t, it);
   }
 }
 
@@ -1195,24 +1213,34 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
   return hipGetLastError();
 }
 
+template <bool TR>
+static void launch_split8(const BootArgs& a, int is_goldilocks, hipStream_t s) {
+  const int db = is_goldilocks ? dig_bits(a) : 0;
+  if (db == 8)
+    hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 8, TR>), dim3(a.count), dim3(512), 0, s, a);
+  else if (db == 5)
+    hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 5, TR>), dim3(a.count), dim3(512), 0, s, a);
+  else if (is_goldilocks)
+    hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 0, TR>), dim3(a.count), dim3(512), 0, s, a);
+  else
+    hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8, 0, TR>), dim3(a.count), dim3(512), 0, s, a);
+}
+
 hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
                                   hipStream_t s) {
   if (waves != 4 && waves != 8) return hipErrorInvalidValue;
   if (waves == 4) {
+    if (a.trace) return hipErrorInvalidValue;  // traced gates need 8 waves
     if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4, 0>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 4, 0, false>), dim3(a.count), dim3(256), 0,
+                         s, a);
     else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4, 0>), dim3(a.count), dim3(256), 0, s, a);
+      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 4, 0, false>), dim3(a.count), dim3(256), 0,
+                         s, a);
+  } else if (a.trace) {
+    launch_split8<true>(a, is_goldilocks, s);
   } else {
-    const int db = is_goldilocks ? dig_bits(a) : 0;
-    if (db == 8)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 8>), dim3(a.count), dim3(512), 0, s, a);
-    else if (db == 5)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 5>), dim3(a.count), dim3(512), 0, s, a);
-    else if (is_goldilocks)
-      hipLaunchKernelGGL((bootstrap_split_kernel<Gl, 8, 0>), dim3(a.count), dim3(512), 0, s, a);
-    else
-      hipLaunchKernelGGL((bootstrap_split_kernel<Bb, 8, 0>), dim3(a.count), dim3(512), 0, s, a);
+    launch_split8<false>(a, is_goldilocks, s);
   }
   return hipGetLastError();
 }
@@ -1228,14 +1256,19 @@ hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int w
 }
 
 hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_gates, int waves,
-                              hipStream_t s) {
+                              bool traced, hipStream_t s) {
   if (waves != 4 && waves != 8) return hipErrorInvalidValue;
-  if (waves == 4)
-    hipLaunchKernelGGL(mixed_split_kernel<4>, dim3(total_gates), dim3(256), 0, s, parts_dev,
-                       nparts);
-  else
-    hipLaunchKernelGGL(mixed_split_kernel<8>, dim3(total_gates), dim3(512), 0, s, parts_dev,
-                       nparts);
+  if (waves == 4) {
+    if (traced) return hipErrorInvalidValue;  // traced gates need 8 waves
+    hipLaunchKernelGGL((mixed_split_kernel<4, false>), dim3(total_gates), dim3(256), 0, s,
+                       parts_dev, nparts);
+  } else if (traced) {
+    hipLaunchKernelGGL((mixed_split_kernel<8, true>), dim3(total_gates), dim3(512), 0, s,
+                       parts_dev, nparts);
+  } else {
+    hipLaunchKernelGGL((mixed_split_kernel<8, false>), dim3(total_gates), dim3(512), 0, s,
+                       parts_dev, nparts);
+  }
   return hipGetLastError();
 }
 

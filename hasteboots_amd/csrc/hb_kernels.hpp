@@ -57,13 +57,14 @@ constexpr size_t kScratchWordsPerGate = 4 * 1024;
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2), (4,3)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
-// Latency mode: `waves` (4 or 8) wavefronts per gate (untraced gates only).
+// Latency mode: `waves` (4 or 8) wavefronts per gate (traced gates: 8 only).
 hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
                                   hipStream_t s);
 // One launch over `nparts` parts (device array of BootArgs, parts in block order).
-// Latency-mode mixed launch (`waves` = 4 or 8 wavefronts per gate; untraced parts only).
+// Latency-mode mixed launch (`waves` = 4 or 8 wavefronts per gate; `traced`: some part is
+// traced, 8 waves only).
 hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
-                              int waves, hipStream_t s);
+                              int waves, bool traced, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
 // Throughput-mode mixed launch: kMixedBatchGpw gates of one part per workgroup, two waves

@@ -274,12 +274,13 @@ int launch(hb_server* S, Slot& sl) {
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
   for (const auto& w : sl.work) sets |= 1 << w.first->ek->p.set;
-  const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced);
+  const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced, narrow);
   const uint32_t blocks = assign_blocks(sl.parts_host, nparts, mixed_gpw(kind));
   HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
-  HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sets, sl.s));
+  HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sets, sl.s,
+                              any_traced));
   HIP_TRY(hipEventRecord(sl.ev1, sl.s));
   sl.busy = true;
   sl.gates = used;
