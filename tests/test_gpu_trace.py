@@ -19,8 +19,9 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("set_id,rounds", [(0, [0, 1, 2, 300, 726, 727]), (1, [0, 1, 727])],
-                         ids=["babybear", "goldilocks"])
+@pytest.mark.parametrize("set_id,rounds",
+                         [(0, [0, 1, 2, 300, 726, 727]), (1, [0, 1, 727]), (2, [0, 363, 727])],
+                         ids=["babybear", "goldilocks", "zama"])
 def test_expanded_trace_matches_oracle(hb, orc, keyset, set_id, rounds):
     say(f"trace set={set_id}")
     P, sk, bsk, ksk = keyset(set_id)
@@ -108,3 +109,26 @@ def test_finalized_containers_match_oracle(hb, orc, keyset, set_id):
     bad = [(hb.TRACE_PARTS[p], j) for (p, j), good in res.items() if not good]
     assert not bad, bad
     say(f"{len(res)} containers equal")
+
+
+@pytest.mark.parametrize("set_id", [1, 2, 0], ids=["goldilocks", "zama", "babybear"])
+def test_traced_latency_mode_equals_one_wave(hb, keyset, set_id):
+    """Traced gates in latency mode (the 8-wave kernel writing the history from registers)
+    and with latency mode off (one wavefront per gate, the history as its accumulator) give
+    the same outputs and the same core trace, word for word."""
+    say(f"traced latency mode against one wave, set={set_id}")
+    P, sk, bsk, ksk = keyset(set_id)
+    ev = hb.Evaluator(keys=(bsk, ksk), params=P)
+    enc = hb.Encryptor(sk)
+    rng = stream(70 + set_id, "trace-modes")
+    b = rng.integers(0, 2, size=(2, 5))
+    c0, c1 = enc.encrypt_many(b[0]), enc.encrypt_many(b[1])
+    modes = hb.latency_modes()
+    try:
+        split_out, split_core = ev.gate_batch_traced("nand", c0, c1)
+        hb.set_latency_mode(0)
+        one_out, one_core = ev.gate_batch_traced("nand", c0, c1)
+    finally:
+        hb.restore_latency_modes(modes)
+    assert np.array_equal(split_out, one_out)
+    assert np.array_equal(split_core, one_core)
