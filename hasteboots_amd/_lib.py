@@ -133,6 +133,8 @@ _PROTOS = {
                                           C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "hb_set_latency_mode": (C.c_int, [C.c_uint32]),
     "hb_set_latency_mode_for": (C.c_int, [C.c_int, C.c_uint32]),
+    "hb_set_pair_mode": (C.c_int, [C.c_int]),
+    "hb_pair_mode": (C.c_int, [C.POINTER(C.c_int)]),
     "hb_latency_mode": (C.c_int, [C.POINTER(C.c_uint32)]),
     "hb_latency_mode_for": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "hb_modmul_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double),

@@ -443,18 +443,50 @@ hipError_t launch_shaped(const Engine& E, const ParamSet& p, const BootArgs& a, 
   return launch_bootstrap(a, p.is_goldilocks, gpw, wps, s);
 }
 
+// Pair mode for this batch: Goldilocks with 8-bit digits, within the latency-mode cutoff
+// and one gate per two CUs (bootstrap_gate_pair).
+bool pair_batch(const Engine& E, const ParamSet& p, size_t count) {
+  return E.pair_mode && p.is_goldilocks && p.br_bits == 8 && p.br_len == 8 &&
+         count <= E.split_max[p.set] && count <= E.simd_slots / 8;
+}
+
+int ensure_pair_flags(hb_ek* ek, size_t count) {
+  if (count <= ek->pair_cap) return HB_OK;
+  if (ek->acc_ev_used) HIP_TRY(hipEventSynchronize(ek->acc_ev));
+  if (ek->pair_flags) HIP_TRY(hipFree(ek->pair_flags));
+  ek->pair_flags = nullptr;
+  ek->pair_cap = 0;
+  const size_t bytes = count * kPairFlagWords * sizeof(uint64_t);
+  HIP_TRY(hipMalloc((void**)&ek->pair_flags, bytes));
+  HIP_TRY(hipMemset(ek->pair_flags, 0, bytes));  // epochs start at 1
+  ek->pair_cap = count;
+  return HB_OK;
+}
+
 int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const uint64_t* c1,
                  const uint64_t* c2, const uint64_t* lut_dev, uint64_t* out, hipStream_t s) {
   const ParamSet& p = ek->p;
   Engine& E = engine();
   int rc = ensure_acc(ek, count);
   if (rc) return rc;
+  const bool pair = pair_batch(E, p, count);
+  if (pair) {
+    rc = ensure_pair_flags(ek, count);
+    if (rc) return rc;
+  }
   BootArgs a = make_args(ek, gate, count, c0, c1, c2, lut_dev, out, ek->acc_dev);
   rc = acc_acquire(ek, s);
   if (rc) return rc;
   rc = timer_start(s);
   if (rc) return rc;
-  HIP_TRY(launch_shaped(E, p, a, s));
+  bool done = false;
+  if (pair) {
+    a.pair_flags = ek->pair_flags;
+    a.pair_epoch = ++ek->pair_epoch;
+    done = launch_bootstrap_pair(a, s) == hipSuccess;
+    if (!done) (void)hipGetLastError();  // refused (co-residency): latency mode instead
+  }
+  if (!done) HIP_TRY(launch_shaped(E, p, a, s));
   rc = timer_stop(s);
   if (rc) return rc;
   return acc_release(ek, s);
@@ -664,6 +696,7 @@ int hb_ek_destroy(hb_ek_t* ek) {
   if (ek->acc_ev_used) (void)hipEventSynchronize(ek->acc_ev);
   if (ek->acc_dev) (void)hipFree(ek->acc_dev);
   if (ek->mux_dev) (void)hipFree(ek->mux_dev);
+  if (ek->pair_flags) (void)hipFree(ek->pair_flags);
   if (ek->acc_ev) (void)hipEventDestroy(ek->acc_ev);
   delete ek;
   return HB_OK;
@@ -1307,6 +1340,21 @@ int hb_set_latency_mode_for(int set, uint32_t max_gates) {
   Engine& E = engine();
   E.split_max[set] = max_gates == HB_LATENCY_DEFAULT ? default_split_max(set, E.simd_slots)
                                                      : max_gates;
+  return HB_OK;
+}
+
+int hb_set_pair_mode(int enable) {
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  engine().pair_mode = enable != 0;
+  return HB_OK;
+}
+
+int hb_pair_mode(int* enabled) {
+  if (!enabled) return fail(HB_ERR_ARG, "null argument");
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  *enabled = engine().pair_mode ? 1 : 0;
   return HB_OK;
 }
 

@@ -73,6 +73,9 @@ struct Engine {
   // 103.8, 2048 112.6 / 104.1; Zama 1536 138.5 / 159.0, 1792 161.7 / 160.4; BabyBear 1792
   // 27.8 / 28.5, 2048 31.5 / 28.5.
   uint32_t split_max[3] = {1792, 1792, 1536};
+  // Pair mode (Goldilocks, untraced batches within the latency-mode cutoff and at most one
+  // gate per two CUs): hb_set_pair_mode switches it.
+  bool pair_mode = true;
 };
 
 // Built-in latency-mode cutoff of a parameter set on a device with `simd_slots` SIMDs.
@@ -196,6 +199,11 @@ struct hb_ek {
   // Mux staging (6 rows of n+1 words per gate), guarded by acc_ev like acc_dev.
   uint64_t* mux_dev = nullptr;
   size_t mux_cap = 0;  // gates
+  // Pair mode's flag words (kPairFlagWords per gate, zeroed once) and the epoch of its last
+  // launch, guarded by acc_ev like acc_dev.
+  uint64_t* pair_flags = nullptr;
+  size_t pair_cap = 0;  // gates
+  uint32_t pair_epoch = 0;
 };
 
 namespace hb {

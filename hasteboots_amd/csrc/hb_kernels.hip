@@ -286,16 +286,17 @@ __device__ __forceinline__ void gadget_mac_dig8(u64 (&pa)[E], u64 (&pb)[E],
   }
 }
 
-// Latency mode's share of the Goldilocks 8-bit gadget: digits j0 and j0 + 1 of the rotated
-// component (`dst`, canonical words), keys from row j0 (`key`). Both digits are packed from
-// registers into ntt_fwd_bytes' layout (dword m byte g = the digit of word 4m + g) with four
-// byte permutes per group of four words, as store_digits8 does for all eight. swap: pa
-// takes the b products and pb the a products.
+// Latency mode's share of the Goldilocks 8-bit gadget: digits j0 and j0 + 1 (nd = 2; pair
+// mode: j0 alone, nd = 1) of the rotated component (`dst`, canonical words), keys from row
+// j0 (`key`). The digits are packed from registers into ntt_fwd_bytes' layout (dword m byte
+// g = the digit of word 4m + g) with four byte permutes per group of four words, as
+// store_digits8 does for all eight. swap: pa takes the b products and pb the a products.
 template <bool PF>
 __device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
                                                   const u64 (&dst)[16], u32 j0,
                                                   const u64* __restrict__ key, u64* lds,
-                                                  const u64* twv, int lane, bool swap) {
+                                                  const u64* twv, int lane, bool swap,
+                                                  u32 nd = 2) {
   constexpr int N = 1024;
   const int oa = swap ? N : 0, ob = swap ? 0 : N;
   u32 d[4], dn[4];
@@ -309,7 +310,7 @@ __device__ __forceinline__ void gadget_mac_bytes2(u64 (&pa)[16], u64 (&pb)[16],
     dn[m] = __builtin_amdgcn_perm(t23, t01, 0x07060302u);
   }
 #pragma unroll 1
-  for (u32 j = 0; j < 2; j++) {
+  for (u32 j = 0; j < nd; j++) {
     const u64* ka = key + (size_t)(2 * j) * N + oa;
     const u64* kb = key + (size_t)(2 * j) * N + ob;
     u64 ka0[4], kb0[4];
@@ -852,6 +853,175 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   }
 }
 
+// ----------------------------------------------------------------------------
+// Pair mode (Goldilocks, 8 digits of 8 bits, untraced): one gate on two workgroups of 8
+// wavefronts, blocks 2g (c = 0) and 2g + 1 (c = 1), on two CUs of one cooperative launch.
+// Workgroup c owns accumulator component c: it rotates it ([C | M] in LDS, as
+// bootstrap_gate_split keeps it), wave w runs digit row w of it (one digit per wave, half
+// of a latency-mode wave's rows), and after the balanced partial sums its waves hold the
+// quarters of both output halves. The half of the other component goes to the partner
+// through global memory (scratch, [sender][parity][quarter][e][lane], two parities by
+// exchange count), quarter by quarter, each with an agent-scope release flag; the waves of
+// half c wait for the partner's quarters, add them, and finish half c (cooperative inverse,
+// accumulator update) as latency mode does. Both workgroups skip the same rounds (each
+// computes the modulus switch), so exchange e pairs up. Workgroup 1 hands the b component's
+// word 0 to workgroup 0, which key-switches and writes the output.
+// The exchange runs on agent-scope relaxed atomics, which the compiler emits as sc1 loads
+// and stores, coherent across the XCDs' L2s: the sender stores its quarter, waits for the
+// stores to complete (s_waitcnt vmcnt(0)) and stores the flag; the receiver polls the flag
+// and then loads the quarter, both past its L2. Agent-scope release/acquire fences (L2
+// write-back and invalidation every round) cost 6 ms at 64 gates.
+// Waits poll with relaxed agent-scope loads and end after kPairPolls polls; a timeout
+// records the launch's epoch in flag word 10, which ends the waits of both workgroups,
+// and a wave whose wait failed waits no more (the outputs are then wrong), so every wave
+// reaches the end after at most one timeout.
+// ----------------------------------------------------------------------------
+constexpr u32 kPairPolls = 1u << 16;
+
+__device__ __forceinline__ bool pair_wait(const u64* f, u64 want, u64* abort_word, u64 epoch) {
+  for (u32 it = 0; it < kPairPolls; it++) {
+    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
+    if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+      return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(abort_word, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return false;
+}
+
+__device__ __forceinline__ void pair_put(u64* p, u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 pair_get(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The flag store after every earlier pair_put of the wave has completed.
+__device__ __forceinline__ void pair_signal(u64* f, u64 v) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, int c, u64* lds,
+                                                    int ldsw, unsigned short* ms, u64* part,
+                                                    const u64* twl, const u64* itwl) {
+  using W = u64;
+  constexpr int E = 16;
+  constexpr int N = 64 * E;
+  constexpr bool PF = true;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ch = wave >> 2, qw = wave & 3;  // the output half and quarter this wave finishes
+  const bool fin = ch == c;                 // ... in this workgroup; the others send theirs
+  const u32 n = A.n;
+  const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
+  W* const xch = static_cast<W*>(A.acc) + (size_t)g * kScratchWordsPerGate;
+  u64* const fl = A.pair_flags + (size_t)g * kPairFlagWords;
+  const u64 ep = (u64)A.pair_epoch << 32;
+
+  gate_modswitch<Gl>(A, g, tid, 64 * 8, ms);
+  __syncthreads();
+  gate_lut_init<Gl>(A, tid, 64 * 8, ms, part, part + N);  // both components, raw
+  __syncthreads();
+  // the finishing waves keep the raw words of their positions of component c; [C | M] of
+  // component c in part[0, 2N)
+  W raw[4];
+  if (fin) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) raw[k] = part[(c ? N : 0) + 64 * qw + lane + 256 * k];
+  }
+  __syncthreads();
+  if (fin) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int pos = 64 * qw + lane + 256 * k;
+      part[pos] = Gl::canon(raw[k]);
+      part[N + pos] = Gl::canon(Gl::neg_raw(raw[k]));
+    }
+  }
+  __syncthreads();
+
+  const size_t rgsw_words = (size_t)2 * A.br_len * 2 * N;
+  W* mine = part + (size_t)(wave & 3) * 2 * N;
+  u32 ex = 0;         // exchanges so far
+  bool alive = true;  // no wait of this wave has timed out
+  for (u32 i = 0; i < n; i++) {
+    const int r = ms[i];
+    if (r == 0) continue;  // uniform across both workgroups
+    const W* key = bsk + (size_t)i * rgsw_words + (size_t)c * A.br_len * 2 * N;
+    W pa[E], pb[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
+    {
+      W dst[E];
+      rotate_sub_cn<Gl, E>(part, r, lane, dst);
+      gadget_mac_bytes2<PF>(pa, pb, dst, (u32)wave, key + (size_t)wave * 2 * N, lds, twl, lane,
+                            wave >= 4, 1);
+    }
+    // balanced partial sums (bootstrap_gate_split with 8 waves)
+#pragma unroll
+    for (int e = 0; e < E; e++) lds[e * 64 + lane] = pb[e];
+    __syncthreads();
+    {
+      const W* take = lds + (ptrdiff_t)(wave < 4 ? 4 : -4) * ldsw;
+      W* put = mine + (wave < 4 ? 0 : N);
+#pragma unroll
+      for (int e = 0; e < E; e++) put[e * 64 + lane] = DF<Gl>::add(pa[e], take[e * 64 + lane]);
+    }
+    __syncthreads();
+    W x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = 4 * qw + k;
+      W v = part[(size_t)ch * N + e * 64 + lane];
+#pragma unroll
+      for (int w = 1; w < 4; w++) v = DF<Gl>::add(v, part[(size_t)w * 2 * N + ch * N + e * 64 + lane]);
+      x[k] = v;
+    }
+    const u64 want = ep | (u64)(ex + 1);
+    const size_t par = ex & 1;
+    if (!fin) {
+      W* to = xch + ((size_t)(c * 2 + par) * 4 + qw) * 256;
+#pragma unroll
+      for (int k = 0; k < 4; k++) pair_put(to + k * 64 + lane, x[k]);
+      pair_signal(fl + c * 4 + qw, want);
+    } else {
+      alive = alive && pair_wait(fl + (1 - c) * 4 + qw, want, fl + 10, ep);
+      if (alive) {
+        const W* from = xch + ((size_t)((1 - c) * 2 + par) * 4 + qw) * 256;
+#pragma unroll
+        for (int k = 0; k < 4; k++) x[k] = DF<Gl>::add(x[k], pair_get(from + k * 64 + lane));
+      }
+      ntt_inv_quarter<Gl>(x, lds, itwl, lane, qw);
+    }
+    __syncthreads();
+    if (fin) {
+      ntt_inv_cross<Gl>(x, lds - (ptrdiff_t)qw * ldsw, ldsw, itwl, lane, qw);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int pos = 64 * qw + lane + 256 * k;
+        raw[k] = Gl::add_cy(raw[k], x[k]);
+        part[pos] = Gl::canon(raw[k]);
+        part[N + pos] = Gl::canon(Gl::neg_raw(raw[k]));
+      }
+    }
+    __syncthreads();
+    ex++;
+  }
+  // key switch in workgroup 0 with the b component's word 0 from workgroup 1 (C[0] serves
+  // for the raw word: keyswitch_extract canonicalises b0)
+  if (wave == 0) {
+    if (c == 1) {
+      pair_put(fl + 8, part[0]);
+      pair_signal(fl + 9, ep | 1);
+    } else {
+      W b0 = 0;
+      if (alive && pair_wait(fl + 9, ep | 1, fl + 10, ep)) b0 = pair_get(fl + 8);
+      part[2 * N + lane] = b0;
+      wave_sync();
+      keyswitch_extract<Gl, PF, true>(A, g, part + N, part + 2 * N, lds, twl, itwl, lane);
+    }
+  }
+}
+
 // Copies the forward and inverse twiddle tables (N words each; the Goldilocks forward table
 // of the digit transforms, ntt_fwd_digits, has kTwxWords) into LDS for the per-lane reads of
 // NTT passes 2 and 3. twl holds kTwxWords words.
@@ -905,6 +1075,18 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
   W* const ab = WAVES == 4 ? accl + 1024 : part + 2 * 1024;
   bootstrap_gate_split<F, WAVES, DIG, TR>(A, blockIdx.x, lds[threadIdx.x >> 6], 1024 + 64, ms,
                                        part, aa, ab, twl, itwl);
+}
+
+// Pair-mode launch: workgroups 2g and 2g + 1 carry gate g (bootstrap_gate_pair).
+__global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
+  __shared__ u64 lds[8][1024 + 64];
+  __shared__ unsigned short ms[1024];
+  __shared__ u64 twl[kTwxWords];
+  __shared__ u64 itwl[1024];
+  __shared__ u64 part[4 * 2 * 1024];
+  stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
+  bootstrap_gate_pair(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1), lds[threadIdx.x >> 6],
+                      1024 + 64, ms, part, twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -1211,6 +1393,15 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s) {
+  if (dig_bits(a) != 8 || a.trace || a.idx || !a.pair_flags || a.count == 0)
+    return hipErrorInvalidValue;
+  BootArgs arg = a;
+  void* params[] = {&arg};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&bootstrap_pair_kernel),
+                                    dim3(2 * a.count), dim3(64 * 8), params, 0, s);
 }
 
 template <bool TR>

@@ -46,7 +46,14 @@ struct BootArgs {
   uint32_t first_block;  // mixed launches: first workgroup index of this part
   const uint32_t* idx;   // optional: per gate (in0, in1, out) wire rows of c0 / out
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
+  uint64_t* pair_flags;  // pair mode: count * kPairFlagWords words, zeroed once
+  uint32_t pair_epoch;   // pair mode: this launch's epoch (> every earlier one on pair_flags)
 };
+
+// Pair mode's per-gate flag words (launch_bootstrap_pair): [0, 8) the quarter flags of the
+// two workgroups, 8 the b component's word 0 for the key switch, 9 its flag, 10 the epoch
+// of a launch whose wait timed out.
+constexpr int kPairFlagWords = 16;
 
 // Per-gate scratch in BootArgs::acc, in words of the set's size: the accumulator RLWE (2N
 // words) then the rotated accumulator's byte-sized gadget digits, byte-transposed (one
@@ -57,6 +64,10 @@ constexpr size_t kScratchWordsPerGate = 4 * 1024;
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2), (4,3)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
+// Pair mode (Goldilocks with 8-bit digits, untraced): one gate on two workgroups of 8
+// waves, on two CUs, one accumulator component each, as one cooperative launch of 2 * count
+// workgroups; the caller falls back to latency mode when the launch is refused.
+hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s);
 // Latency mode: `waves` (4 or 8) wavefronts per gate (traced gates: 8 only).
 hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
                                   hipStream_t s);

@@ -545,6 +545,17 @@ def restore_latency_modes(modes) -> None:
         set_latency_mode(m, s)
 
 
+def pair_mode() -> bool:
+    """Whether small Goldilocks batches run one gate on two compute units (hb_pair_mode)."""
+    v = C.c_int()
+    check(lib().hb_pair_mode(C.byref(v)))
+    return bool(v.value)
+
+
+def set_pair_mode(enable: bool) -> None:
+    check(lib().hb_set_pair_mode(1 if enable else 0))
+
+
 def mixed_batch(parts) -> list[np.ndarray]:
     """Bootstrap gates of several evaluation keys (any parameter sets) in ONE launch.
 

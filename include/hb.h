@@ -220,6 +220,16 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * one; 0 disables latency mode, HB_LATENCY_DEFAULT restores the built-in cutoff.
  * hb_latency_mode reports the Goldilocks cutoff, hb_latency_mode_for any set's. */
 #define HB_LATENCY_DEFAULT 0xFFFFFFFFu
+/* Pair mode (no reference counterpart): Goldilocks batches (8-bit gadget, untraced) within
+ * the latency-mode cutoff and of at most one gate per two compute units (128 on MI355X) run
+ * each gate on two workgroups of 8 wavefronts on two compute units, one accumulator
+ * component each, exchanging half of the external product through global memory every
+ * round (a cooperative launch; where the device refuses it, latency mode runs instead).
+ * Calls through hb_gate_batch* and the single-gate calls take it; captured graphs, the
+ * no-allocation call, mixed launches and the serving loop keep latency mode. On by
+ * default; hb_set_pair_mode(0) turns it off. Same output words as latency mode. */
+int hb_set_pair_mode(int enable);
+int hb_pair_mode(int* enabled);
 int hb_set_latency_mode(uint32_t max_gates);
 int hb_set_latency_mode_for(int set, uint32_t max_gates);
 int hb_latency_mode(uint32_t* max_gates);

@@ -222,3 +222,26 @@ def test_partly_filled_round_runs_its_rest_in_latency_mode(hb, orc, gold):
     sample = np.r_[2040:2056, count - 4:count]  # both sides of the split
     ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=12)
     assert np.array_equal(shaped[sample], ref)
+
+
+@pytest.mark.parametrize("count", [1, 7, 128])
+def test_pair_mode_matches_latency_mode(hb, orc, gold, count):
+    """Goldilocks gates on two CUs each (pair mode: the external product's halves cross
+    through global memory every round) give the latency-mode words, and the oracle's."""
+    P, sk, bsk, ksk, ev = gold
+    say(f"pair mode, {count} gates")
+    b0, b1, c0, c1 = _inputs(hb, sk, 80, count)
+    assert hb.pair_mode()
+    paired = ev.gate_batch("nand", c0, c1)
+    hb.set_pair_mode(False)
+    try:
+        split = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_pair_mode(True)
+    assert np.array_equal(paired, split)
+    sample = np.unique(np.r_[0:min(2, count), max(0, count - 2):count])
+    ref = orc.OracleKeys(1, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=4)
+    assert np.array_equal(paired[sample], ref)
+    dec = hb.Decryptor(sk)
+    assert all(dec.decrypt(hb.LWECiphertext(paired[i])) == 1 - (int(b0[i]) & int(b1[i]))
+               for i in range(count))
