@@ -854,11 +854,12 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 }
 
 // ----------------------------------------------------------------------------
-// Pair mode (Goldilocks, 8 digits of 8 bits, untraced): one gate on two workgroups of 8
+// Pair mode (Goldilocks with byte-digit gadgets, untraced): one gate on two workgroups of 8
 // wavefronts, blocks 2g (c = 0) and 2g + 1 (c = 1), on two CUs of one cooperative launch.
 // Workgroup c owns accumulator component c: it rotates it ([C | M] in LDS, as
-// bootstrap_gate_split keeps it), wave w runs digit row w of it (one digit per wave, half
-// of a latency-mode wave's rows), and after the balanced partial sums its waves hold the
+// bootstrap_gate_split keeps it), wave w runs digit row w of it (DIG = 8: one digit per
+// wave, half of a latency-mode wave's rows; DIG = 5: one or two of the 13), and after the
+// balanced partial sums its waves hold the
 // quarters of both output halves. The half of the other component goes to the partner
 // through global memory (scratch, [sender][parity][quarter][e][lane], two parities by
 // exchange count), quarter by quarter, each with an agent-scope release flag; the waves of
@@ -901,9 +902,11 @@ __device__ __forceinline__ void pair_signal(u64* f, u64 v) {
   __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int DIG>
 __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, int c, u64* lds,
                                                     int ldsw, unsigned short* ms, u64* part,
                                                     const u64* twl, const u64* itwl) {
+  static_assert(DIG == 8 || DIG == 5, "byte-digit gadgets");
   using W = u64;
   constexpr int E = 16;
   constexpr int N = 64 * E;
@@ -953,8 +956,19 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
     {
       W dst[E];
       rotate_sub_cn<Gl, E>(part, r, lane, dst);
-      gadget_mac_bytes2<PF>(pa, pb, dst, (u32)wave, key + (size_t)wave * 2 * N, lds, twl, lane,
-                            wave >= 4, 1);
+      if constexpr (DIG == 8) {
+        gadget_mac_bytes2<PF>(pa, pb, dst, (u32)wave, key + (size_t)wave * 2 * N, lds, twl, lane,
+                              wave >= 4, 1);
+      } else {
+        // 13 digits over 8 waves: windows of 1 or 2 digits (1, 2, 1, 2, 2, 1, 2, 2), so that
+        // the two waves of a SIMD (w, w + 4) take 3 or 4 rows
+        const u32 j0 = (u32)wave * 13 / 8, j1 = (u32)(wave + 1) * 13 / 8;
+        u32 win[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (5 * j0));
+        gadget_mac_window_bytes<5, PF>(pa, pb, win, j1 - j0, key + (size_t)j0 * 2 * N, lds, twl,
+                                       lane, wave >= 4);
+      }
     }
     // balanced partial sums (bootstrap_gate_split with 8 waves)
 #pragma unroll
@@ -1078,6 +1092,7 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
 }
 
 // Pair-mode launch: workgroups 2g and 2g + 1 carry gate g (bootstrap_gate_pair).
+template <int DIG>
 __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
   __shared__ u64 lds[8][1024 + 64];
   __shared__ unsigned short ms[1024];
@@ -1085,8 +1100,8 @@ __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
   __shared__ u64 itwl[1024];
   __shared__ u64 part[4 * 2 * 1024];
   stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
-  bootstrap_gate_pair(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1), lds[threadIdx.x >> 6],
-                      1024 + 64, ms, part, twl, itwl);
+  bootstrap_gate_pair<DIG>(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1),
+                           lds[threadIdx.x >> 6], 1024 + 64, ms, part, twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -1396,12 +1411,14 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
 }
 
 hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s) {
-  if (dig_bits(a) != 8 || a.trace || a.idx || !a.pair_flags || a.count == 0)
+  const int db = dig_bits(a);
+  if ((db != 8 && db != 5) || a.trace || a.idx || !a.pair_flags || a.count == 0)
     return hipErrorInvalidValue;
   BootArgs arg = a;
   void* params[] = {&arg};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&bootstrap_pair_kernel),
-                                    dim3(2 * a.count), dim3(64 * 8), params, 0, s);
+  const void* f = db == 8 ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<8>)
+                          : reinterpret_cast<const void*>(&bootstrap_pair_kernel<5>);
+  return hipLaunchCooperativeKernel(f, dim3(2 * a.count), dim3(64 * 8), params, 0, s);
 }
 
 template <bool TR>

@@ -220,7 +220,7 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * one; 0 disables latency mode, HB_LATENCY_DEFAULT restores the built-in cutoff.
  * hb_latency_mode reports the Goldilocks cutoff, hb_latency_mode_for any set's. */
 #define HB_LATENCY_DEFAULT 0xFFFFFFFFu
-/* Pair mode (no reference counterpart): Goldilocks batches (8-bit gadget, untraced) within
+/* Pair mode (no reference counterpart): Goldilocks and Zama batches (untraced) within
  * the latency-mode cutoff and of at most one gate per two compute units (128 on MI355X) run
  * each gate on two workgroups of 8 wavefronts on two compute units, one accumulator
  * component each, exchanging half of the external product through global memory every

@@ -245,3 +245,25 @@ def test_pair_mode_matches_latency_mode(hb, orc, gold, count):
     dec = hb.Decryptor(sk)
     assert all(dec.decrypt(hb.LWECiphertext(paired[i])) == 1 - (int(b0[i]) & int(b1[i]))
                for i in range(count))
+
+
+@pytest.mark.parametrize("count", [1, 96])
+def test_pair_mode_zama_matches_latency_mode(hb, orc, keyset, count):
+    """Zama (13 five-bit digits, one or two per wave) in pair mode: the latency-mode words
+    and the oracle's."""
+    P, sk, bsk, ksk = keyset(2)
+    say(f"pair mode zama, {count} gates")
+    if 2 not in _EVALUATORS:
+        _EVALUATORS[2] = hb.Evaluator(keys=(bsk, ksk), params=P)
+    ev = _EVALUATORS[2]
+    b0, b1, c0, c1 = _inputs_for(hb, sk, 2, 81, count)
+    paired = ev.gate_batch("nand", c0, c1)
+    hb.set_pair_mode(False)
+    try:
+        split = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_pair_mode(True)
+    assert np.array_equal(paired, split)
+    sample = np.unique(np.r_[0:1, count - 1:count])
+    ref = orc.OracleKeys(2, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=2)
+    assert np.array_equal(paired[sample], ref)
