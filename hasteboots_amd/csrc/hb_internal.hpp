@@ -127,7 +127,7 @@ inline int mixed_split_waves(bool narrow) { return narrow ? 8 : 4; }
 // mixes sets: heaviest parts first, the tail workgroups hold the lightest gates), one
 // wavefront per gate up to one gate per SIMD, the (4, 2) batched shape above that.
 // Traced gates take latency mode only where it runs 8 waves (`narrow`).
-enum class MixedKernel { split, one_wave, batch };
+enum class MixedKernel { split, one_wave, batch, pair };
 // Whether every Goldilocks set in `sets` has a byte-digit blind-rotation gadget (8 digits
 // of 8 bits or 13 of 5, as every set of param_set has): the mixed latency-mode and
 // throughput kernels hold only those Goldilocks bodies; any other gadget goes to the
@@ -141,8 +141,12 @@ inline bool byte_digit_sets(int sets) {
   return true;
 }
 inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced,
-                                    bool narrow = true) {
+                                    bool narrow = true, bool allow_pair = false) {
   if (!byte_digit_sets(sets)) return MixedKernel::one_wave;
+  // pair mode: Goldilocks sets only, at most one gate per two CUs
+  if (allow_pair && E.pair_mode && !traced && (sets & (1 << SET_BABYBEAR)) == 0 && gates > 0 &&
+      gates <= E.simd_slots / 8 && gates <= mixed_split_cutoff(E, sets))
+    return MixedKernel::pair;
   const uint32_t cutoff = mixed_split_cutoff(E, sets);
   const bool mixes = (sets & (sets - 1)) != 0;
   // traced gates run the latency-mode kernel with 8 waves only
@@ -167,6 +171,12 @@ inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, 
     case MixedKernel::split:
       return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), traced, s);
     case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
+    case MixedKernel::pair: {
+      const int cls = sets == (1 << SET_GOLDILOCKS) ? 1 : sets == (1 << SET_ZAMA) ? 2 : 0;
+      if (launch_mixed_pair(parts_dev, nparts, blocks, cls, s) == hipSuccess) return hipSuccess;
+      (void)hipGetLastError();  // refused (co-residency): latency mode, same block numbering
+      return launch_mixed_split(parts_dev, nparts, blocks, 8, false, s);
+    }
     default: return launch_mixed_batch(parts_dev, nparts, blocks, mixed_class(sets), s);
   }
 }

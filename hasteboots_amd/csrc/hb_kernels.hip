@@ -877,7 +877,7 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 // and a wave whose wait failed waits no more (the outputs are then wrong), so every wave
 // reaches the end after at most one timeout.
 // ----------------------------------------------------------------------------
-constexpr u32 kPairPolls = 1u << 16;
+constexpr u32 kPairPolls = 1u << 18;
 
 __device__ __forceinline__ bool pair_wait(const u64* f, u64 want, u64* abort_word, u64 epoch) {
   for (u32 it = 0; it < kPairPolls; it++) {
@@ -1102,6 +1102,30 @@ __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
   stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
   bootstrap_gate_pair<DIG>(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1),
                            lds[threadIdx.x >> 6], 1024 + 64, ms, part, twl, itwl);
+}
+
+// Pair-mode mixed launch (the serving loop): workgroups 2b and 2b + 1 carry gate b of the
+// launch, which belongs to the last part whose first_block <= b (every part Goldilocks
+// with a byte-digit gadget, untraced, with its own pair_flags rows). CLS: 1 every part
+// with 8-bit digits, 2 every part with 5-bit digits, 0 either (runtime branch).
+template <int CLS>
+__global__ __launch_bounds__(64 * 8) void mixed_pair_kernel(const BootArgs* __restrict__ parts,
+                                                            int nparts) {
+  __shared__ u64 lds[8][1024 + 64];
+  __shared__ unsigned short ms[1024];
+  __shared__ u64 twl[kTwxWords];
+  __shared__ u64 itwl[1024];
+  __shared__ u64 part[4 * 2 * 1024];
+  const u32 gb = blockIdx.x >> 1;
+  int p = 0;
+  while (p + 1 < nparts && gb >= parts[p + 1].first_block) p++;
+  const BootArgs A = parts[p];
+  const int g = (int)(gb - A.first_block), c = (int)(blockIdx.x & 1), w = threadIdx.x >> 6;
+  stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
+  if (CLS == 1 || (CLS == 0 && dig_bits(A) == 8))
+    bootstrap_gate_pair<8>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
+  else
+    bootstrap_gate_pair<5>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -1478,6 +1502,16 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_g
                        parts_dev, nparts);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_mixed_pair(const BootArgs* parts_dev, int nparts, u32 total_gates, int cls,
+                             hipStream_t s) {
+  if (total_gates == 0) return hipErrorInvalidValue;
+  void* params[] = {&parts_dev, &nparts};
+  const void* f = cls == 1   ? reinterpret_cast<const void*>(&mixed_pair_kernel<1>)
+                  : cls == 2 ? reinterpret_cast<const void*>(&mixed_pair_kernel<2>)
+                             : reinterpret_cast<const void*>(&mixed_pair_kernel<0>);
+  return hipLaunchCooperativeKernel(f, dim3(2 * total_gates), dim3(64 * 8), params, 0, s);
 }
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {

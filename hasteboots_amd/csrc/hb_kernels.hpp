@@ -78,6 +78,11 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t to
                               int waves, bool traced, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
+// Pair-mode mixed launch (cooperative; the parts' first_block count gates, and every part
+// is Goldilocks with a byte-digit gadget, untraced, with pair_flags and pair_epoch set):
+// cls 1 every part 8-bit digits, 2 every part 5-bit, 0 either.
+hipError_t launch_mixed_pair(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
+                             int cls, hipStream_t s);
 // Throughput-mode mixed launch: kMixedBatchGpw gates of one part per workgroup, two waves
 // per SIMD (the batched kernel's (4, 2) shape). first_block counts workgroups here, so
 // part k occupies ceil(count_k / kMixedBatchGpw) workgroups from its first_block.

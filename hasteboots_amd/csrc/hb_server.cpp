@@ -109,6 +109,8 @@ struct Slot {
   BootArgs* parts_dev = nullptr;
   BootArgs* parts_host = nullptr;  // pinned
   void* acc = nullptr;
+  uint64_t* pair_flags = nullptr;  // pair mode: kPairFlagWords per gate, zeroed once
+  uint32_t pair_epoch = 0;
   bool busy = false;
   std::vector<std::pair<Request*, uint32_t>> work;
   uint32_t gates = 0;
@@ -274,8 +276,20 @@ int launch(hb_server* S, Slot& sl) {
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
   for (const auto& w : sl.work) sets |= 1 << w.first->ek->p.set;
-  const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced, narrow);
+  // pair mode only while the other slot has no kernel in flight: a pair launch needs all
+  // its workgroups resident at once
+  bool other_busy = false;
+  for (const Slot& o : S->slots) other_busy = other_busy || (&o != &sl && o.busy);
+  const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced, narrow,
+                                            !other_busy && sl.pair_flags != nullptr);
   const uint32_t blocks = assign_blocks(sl.parts_host, nparts, mixed_gpw(kind));
+  if (kind == MixedKernel::pair) {
+    ++sl.pair_epoch;
+    for (uint32_t k = 0; k < nparts; k++) {
+      sl.parts_host[k].pair_flags = sl.pair_flags + (size_t)sl.parts_host[k].first_block * kPairFlagWords;
+      sl.parts_host[k].pair_epoch = sl.pair_epoch;
+    }
+  }
   HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
@@ -394,6 +408,7 @@ void release_slot(Slot& sl) {
   if (sl.parts_dev) (void)hipFree(sl.parts_dev);
   if (sl.parts_host) (void)hipHostFree(sl.parts_host);
   if (sl.acc) (void)hipFree(sl.acc);
+  if (sl.pair_flags) (void)hipFree(sl.pair_flags);
   if (sl.ev0) (void)hipEventDestroy(sl.ev0);
   if (sl.ev1) (void)hipEventDestroy(sl.ev1);
   if (sl.s) (void)hipStreamDestroy(sl.s);
@@ -411,6 +426,9 @@ int server_alloc(hb_server* S, uint64_t hbm_budget, uint32_t max_gates) {
     HIP_TRY(hipMalloc((void**)&sl.parts_dev, kMaxParts * sizeof(BootArgs)));
     HIP_TRY(hipHostMalloc((void**)&sl.parts_host, kMaxParts * sizeof(BootArgs)));
     HIP_TRY(hipMalloc(&sl.acc, (size_t)max_gates * kAccPerGate));
+    const size_t pair_bytes = (size_t)max_gates * kPairFlagWords * sizeof(uint64_t);
+    HIP_TRY(hipMalloc((void**)&sl.pair_flags, pair_bytes));
+    HIP_TRY(hipMemset(sl.pair_flags, 0, pair_bytes));  // epochs start at 1
   }
   HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
   return HB_OK;
