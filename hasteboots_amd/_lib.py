@@ -47,7 +47,8 @@ class HbServerStats(C.Structure):
                 ("gates_done", C.c_uint64), ("launches", C.c_uint64), ("queued", C.c_uint64),
                 ("active", C.c_uint64), ("bytes_in_use", C.c_uint64), ("budget", C.c_uint64),
                 ("busy_ms", C.c_double), ("gate_p50_ms", C.c_double), ("gate_p99_ms", C.c_double),
-                ("batched_launches", C.c_uint64), ("max_launch_gates", C.c_uint64)]
+                ("batched_launches", C.c_uint64), ("max_launch_gates", C.c_uint64),
+                ("pair_launches", C.c_uint64)]
 
 
 class HbTuneResult(C.Structure):

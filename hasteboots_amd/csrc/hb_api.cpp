@@ -443,12 +443,14 @@ hipError_t launch_shaped(const Engine& E, const ParamSet& p, const BootArgs& a, 
   return launch_bootstrap(a, p.is_goldilocks, gpw, wps, s);
 }
 
-// Pair mode for this batch: Goldilocks with a byte-digit gadget (8 x 8 bits, 13 x 5 bits),
-// within the latency-mode cutoff and one gate per two CUs (bootstrap_gate_pair).
+// Pair mode for this batch: Goldilocks with a byte-digit gadget (8 x 8 bits, 13 x 5 bits)
+// or BabyBear (at most 4 digits), within the latency-mode cutoff and one gate per two CUs
+// (bootstrap_gate_pair).
 bool pair_batch(const Engine& E, const ParamSet& p, size_t count) {
-  const bool bytes = (p.br_bits == 8 && p.br_len == 8) || (p.br_bits == 5 && p.br_len == 13);
-  return E.pair_mode && p.is_goldilocks && bytes && count <= E.split_max[p.set] &&
-         count <= E.simd_slots / 8;
+  const bool ok = p.is_goldilocks
+                      ? (p.br_bits == 8 && p.br_len == 8) || (p.br_bits == 5 && p.br_len == 13)
+                      : p.br_len <= 4;
+  return E.pair_mode && ok && count <= E.split_max[p.set] && count <= E.simd_slots / 8;
 }
 
 int ensure_pair_flags(hb_ek* ek, size_t count) {

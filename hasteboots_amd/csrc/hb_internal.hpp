@@ -143,8 +143,8 @@ inline bool byte_digit_sets(int sets) {
 inline MixedKernel mixed_kernel_for(const Engine& E, int sets, uint32_t gates, bool traced,
                                     bool narrow = true, bool allow_pair = false) {
   if (!byte_digit_sets(sets)) return MixedKernel::one_wave;
-  // pair mode: Goldilocks sets only, at most one gate per two CUs
-  if (allow_pair && E.pair_mode && !traced && (sets & (1 << SET_BABYBEAR)) == 0 && gates > 0 &&
+  // pair mode: at most one gate per two CUs
+  if (allow_pair && E.pair_mode && !traced && gates > 0 &&
       gates <= E.simd_slots / 8 && gates <= mixed_split_cutoff(E, sets))
     return MixedKernel::pair;
   const uint32_t cutoff = mixed_split_cutoff(E, sets);
@@ -172,7 +172,7 @@ inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, 
       return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), traced, s);
     case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
     case MixedKernel::pair: {
-      const int cls = sets == (1 << SET_GOLDILOCKS) ? 1 : sets == (1 << SET_ZAMA) ? 2 : 0;
+      const int cls = mixed_class(sets);  // 1, 2, 3 one set, 0 a mix
       if (launch_mixed_pair(parts_dev, nparts, blocks, cls, s) == hipSuccess) return hipSuccess;
       (void)hipGetLastError();  // refused (co-residency): latency mode, same block numbering
       return launch_mixed_split(parts_dev, nparts, blocks, 8, false, s);

@@ -896,18 +896,26 @@ __device__ __forceinline__ void pair_put(u64* p, u64 v) {
 __device__ __forceinline__ u64 pair_get(const u64* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void pair_put(u32* p, u32 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u32 pair_get(const u32* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 // The flag store after every earlier pair_put of the wave has completed.
 __device__ __forceinline__ void pair_signal(u64* f, u64 v) {
   __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int DIG>
-__device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, int c, u64* lds,
-                                                    int ldsw, unsigned short* ms, u64* part,
-                                                    const u64* twl, const u64* itwl) {
-  static_assert(DIG == 8 || DIG == 5, "byte-digit gadgets");
-  using W = u64;
+template <class F, int DIG>
+__device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, int c,
+                                                    typename DF<F>::W* lds, int ldsw,
+                                                    unsigned short* ms, typename DF<F>::W* part,
+                                                    const typename DF<F>::W* twl,
+                                                    const typename DF<F>::W* itwl) {
+  static_assert(F::IS_GOLDILOCKS ? (DIG == 8 || DIG == 5) : DIG == 0, "gadget of the field");
+  using W = typename DF<F>::W;
   constexpr int E = 16;
   constexpr int N = 64 * E;
   constexpr bool PF = true;
@@ -920,9 +928,9 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
   u64* const fl = A.pair_flags + (size_t)g * kPairFlagWords;
   const u64 ep = (u64)A.pair_epoch << 32;
 
-  gate_modswitch<Gl>(A, g, tid, 64 * 8, ms);
+  gate_modswitch<F>(A, g, tid, 64 * 8, ms);
   __syncthreads();
-  gate_lut_init<Gl>(A, tid, 64 * 8, ms, part, part + N);  // both components, raw
+  gate_lut_init<F>(A, tid, 64 * 8, ms, part, part + N);  // both components, raw
   __syncthreads();
   // the finishing waves keep the raw words of their positions of component c; [C | M] of
   // component c in part[0, 2N)
@@ -936,8 +944,8 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       const int pos = 64 * qw + lane + 256 * k;
-      part[pos] = Gl::canon(raw[k]);
-      part[N + pos] = Gl::canon(Gl::neg_raw(raw[k]));
+      part[pos] = F::canon(raw[k]);
+      part[N + pos] = F::canon(F::neg_raw(raw[k]));
     }
   }
   __syncthreads();
@@ -955,8 +963,17 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
     for (int e = 0; e < E; e++) pa[e] = pb[e] = 0;
     {
       W dst[E];
-      rotate_sub_cn<Gl, E>(part, r, lane, dst);
-      if constexpr (DIG == 8) {
+      rotate_sub_cn<F, E>(part, r, lane, dst);
+      if constexpr (!F::IS_GOLDILOCKS) {
+        // BabyBear's 4 digits: waves 0-3 one each, so that every SIMD runs one row
+        const u32 len = (u32)wave < A.br_len ? 1u : 0u;
+        const u32 j0 = len ? (u32)wave : 0u;
+        u32 win[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) win[e] = (u32)(dst[e] >> (j0 * A.br_bits));
+        gadget_mac<F, E, PF>(pa, pb, win, len, A.br_bits, key + (size_t)j0 * 2 * N, lds, twl,
+                             lane, wave >= 4);
+      } else if constexpr (DIG == 8) {
         gadget_mac_bytes2<PF>(pa, pb, dst, (u32)wave, key + (size_t)wave * 2 * N, lds, twl, lane,
                               wave >= 4, 1);
       } else {
@@ -978,7 +995,7 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
       const W* take = lds + (ptrdiff_t)(wave < 4 ? 4 : -4) * ldsw;
       W* put = mine + (wave < 4 ? 0 : N);
 #pragma unroll
-      for (int e = 0; e < E; e++) put[e * 64 + lane] = DF<Gl>::add(pa[e], take[e * 64 + lane]);
+      for (int e = 0; e < E; e++) put[e * 64 + lane] = DF<F>::add(pa[e], take[e * 64 + lane]);
     }
     __syncthreads();
     W x[4];
@@ -987,7 +1004,7 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
       const int e = 4 * qw + k;
       W v = part[(size_t)ch * N + e * 64 + lane];
 #pragma unroll
-      for (int w = 1; w < 4; w++) v = DF<Gl>::add(v, part[(size_t)w * 2 * N + ch * N + e * 64 + lane]);
+      for (int w = 1; w < 4; w++) v = DF<F>::add(v, part[(size_t)w * 2 * N + ch * N + e * 64 + lane]);
       x[k] = v;
     }
     const u64 want = ep | (u64)(ex + 1);
@@ -1002,19 +1019,19 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
       if (alive) {
         const W* from = xch + ((size_t)((1 - c) * 2 + par) * 4 + qw) * 256;
 #pragma unroll
-        for (int k = 0; k < 4; k++) x[k] = DF<Gl>::add(x[k], pair_get(from + k * 64 + lane));
+        for (int k = 0; k < 4; k++) x[k] = DF<F>::add(x[k], pair_get(from + k * 64 + lane));
       }
-      ntt_inv_quarter<Gl>(x, lds, itwl, lane, qw);
+      ntt_inv_quarter<F>(x, lds, itwl, lane, qw);
     }
     __syncthreads();
     if (fin) {
-      ntt_inv_cross<Gl>(x, lds - (ptrdiff_t)qw * ldsw, ldsw, itwl, lane, qw);
+      ntt_inv_cross<F>(x, lds - (ptrdiff_t)qw * ldsw, ldsw, itwl, lane, qw);
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         const int pos = 64 * qw + lane + 256 * k;
-        raw[k] = Gl::add_cy(raw[k], x[k]);
-        part[pos] = Gl::canon(raw[k]);
-        part[N + pos] = Gl::canon(Gl::neg_raw(raw[k]));
+        raw[k] = F::add_cy(raw[k], x[k]);
+        part[pos] = F::canon(raw[k]);
+        part[N + pos] = F::canon(F::neg_raw(raw[k]));
       }
     }
     __syncthreads();
@@ -1024,14 +1041,14 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
   // for the raw word: keyswitch_extract canonicalises b0)
   if (wave == 0) {
     if (c == 1) {
-      pair_put(fl + 8, part[0]);
+      pair_put(fl + 8, (u64)part[0]);
       pair_signal(fl + 9, ep | 1);
     } else {
       W b0 = 0;
-      if (alive && pair_wait(fl + 9, ep | 1, fl + 10, ep)) b0 = pair_get(fl + 8);
+      if (alive && pair_wait(fl + 9, ep | 1, fl + 10, ep)) b0 = (W)pair_get(fl + 8);
       part[2 * N + lane] = b0;
       wave_sync();
-      keyswitch_extract<Gl, PF, true>(A, g, part + N, part + 2 * N, lds, twl, itwl, lane);
+      keyswitch_extract<F, PF, true>(A, g, part + N, part + 2 * N, lds, twl, itwl, lane);
     }
   }
 }
@@ -1092,22 +1109,24 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
 }
 
 // Pair-mode launch: workgroups 2g and 2g + 1 carry gate g (bootstrap_gate_pair).
-template <int DIG>
+template <class F, int DIG>
 __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
-  __shared__ u64 lds[8][1024 + 64];
+  using W = typename DF<F>::W;
+  __shared__ W lds[8][1024 + 64];
   __shared__ unsigned short ms[1024];
-  __shared__ u64 twl[kTwxWords];
-  __shared__ u64 itwl[1024];
-  __shared__ u64 part[4 * 2 * 1024];
-  stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
-  bootstrap_gate_pair<DIG>(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1),
-                           lds[threadIdx.x >> 6], 1024 + 64, ms, part, twl, itwl);
+  __shared__ W twl[kTwxWords];
+  __shared__ W itwl[1024];
+  __shared__ W part[4 * 2 * 1024];
+  stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
+  bootstrap_gate_pair<F, DIG>(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1),
+                              lds[threadIdx.x >> 6], 1024 + 64, ms, part, twl, itwl);
 }
 
 // Pair-mode mixed launch (the serving loop): workgroups 2b and 2b + 1 carry gate b of the
-// launch, which belongs to the last part whose first_block <= b (every part Goldilocks
-// with a byte-digit gadget, untraced, with its own pair_flags rows). CLS: 1 every part
-// with 8-bit digits, 2 every part with 5-bit digits, 0 either (runtime branch).
+// launch, which belongs to the last part whose first_block <= b (every part BabyBear with
+// at most 4 digits or Goldilocks with a byte-digit gadget, untraced, with its own
+// pair_flags rows). CLS: 1 every part Goldilocks with 8-bit digits, 2 every part with
+// 5-bit digits, 3 every part BabyBear, 0 any of them (runtime branch).
 template <int CLS>
 __global__ __launch_bounds__(64 * 8) void mixed_pair_kernel(const BootArgs* __restrict__ parts,
                                                             int nparts) {
@@ -1121,11 +1140,25 @@ __global__ __launch_bounds__(64 * 8) void mixed_pair_kernel(const BootArgs* __re
   while (p + 1 < nparts && gb >= parts[p + 1].first_block) p++;
   const BootArgs A = parts[p];
   const int g = (int)(gb - A.first_block), c = (int)(blockIdx.x & 1), w = threadIdx.x >> 6;
-  stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
-  if (CLS == 1 || (CLS == 0 && dig_bits(A) == 8))
-    bootstrap_gate_pair<8>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
-  else
-    bootstrap_gate_pair<5>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
+  if (CLS == 3 || (CLS == 0 && !A.is_goldilocks)) {
+    if constexpr (CLS == 0 || CLS == 3) {
+      // BabyBear: the same LDS arrays in 32-bit words
+      u32* t = reinterpret_cast<u32*>(twl);
+      u32* it = reinterpret_cast<u32*>(itwl);
+      stage_twiddles<u32>(A.tw_fwd, A.tw_inv, t, it, 1024, threadIdx.x, 64 * 8);
+      // the row stride in 32-bit words: twice the u64 row
+      bootstrap_gate_pair<Bb, 0>(A, g, c, reinterpret_cast<u32*>(lds[w]), 2 * (1024 + 64), ms,
+                                 reinterpret_cast<u32*>(part), t, it);
+    }
+    return;
+  }
+  if constexpr (CLS != 3) {
+    stage_twiddles<u64>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
+    if (CLS == 1 || (CLS == 0 && dig_bits(A) == 8))
+      bootstrap_gate_pair<Gl, 8>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
+    else
+      bootstrap_gate_pair<Gl, 5>(A, g, c, lds[w], 1024 + 64, ms, part, twl, itwl);
+  }
 }
 
 // Mixed-parameter launch: workgroup b belongs to the last part whose first_block <= b.
@@ -1436,12 +1469,14 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
 
 hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s) {
   const int db = dig_bits(a);
-  if ((db != 8 && db != 5) || a.trace || a.idx || !a.pair_flags || a.count == 0)
+  const bool bb = !a.is_goldilocks && a.br_len <= 4;  // one digit row per wave 0-3
+  if ((db != 8 && db != 5 && !bb) || a.trace || a.idx || !a.pair_flags || a.count == 0)
     return hipErrorInvalidValue;
   BootArgs arg = a;
   void* params[] = {&arg};
-  const void* f = db == 8 ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<8>)
-                          : reinterpret_cast<const void*>(&bootstrap_pair_kernel<5>);
+  const void* f = bb        ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<Bb, 0>)
+                  : db == 8 ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<Gl, 8>)
+                            : reinterpret_cast<const void*>(&bootstrap_pair_kernel<Gl, 5>);
   return hipLaunchCooperativeKernel(f, dim3(2 * a.count), dim3(64 * 8), params, 0, s);
 }
 
@@ -1510,6 +1545,7 @@ hipError_t launch_mixed_pair(const BootArgs* parts_dev, int nparts, u32 total_ga
   void* params[] = {&parts_dev, &nparts};
   const void* f = cls == 1   ? reinterpret_cast<const void*>(&mixed_pair_kernel<1>)
                   : cls == 2 ? reinterpret_cast<const void*>(&mixed_pair_kernel<2>)
+                  : cls == 3 ? reinterpret_cast<const void*>(&mixed_pair_kernel<3>)
                              : reinterpret_cast<const void*>(&mixed_pair_kernel<0>);
   return hipLaunchCooperativeKernel(f, dim3(2 * total_gates), dim3(64 * 8), params, 0, s);
 }

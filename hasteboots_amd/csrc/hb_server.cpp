@@ -301,6 +301,7 @@ int launch(hb_server* S, Slot& sl) {
   sl.seq = ++S->launch_seq;
   S->st.launches++;
   if (kind == MixedKernel::batch) S->st.batched_launches++;
+  if (kind == MixedKernel::pair) S->st.pair_launches++;
   S->st.max_launch_gates = std::max<uint64_t>(S->st.max_launch_gates, used);
   return HB_OK;
 }

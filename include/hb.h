@@ -220,14 +220,15 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * one; 0 disables latency mode, HB_LATENCY_DEFAULT restores the built-in cutoff.
  * hb_latency_mode reports the Goldilocks cutoff, hb_latency_mode_for any set's. */
 #define HB_LATENCY_DEFAULT 0xFFFFFFFFu
-/* Pair mode (no reference counterpart): Goldilocks and Zama batches (untraced) within
+/* Pair mode (no reference counterpart): batches of every set (untraced) within
  * the latency-mode cutoff and of at most one gate per two compute units (128 on MI355X) run
  * each gate on two workgroups of 8 wavefronts on two compute units, one accumulator
  * component each, exchanging half of the external product through global memory every
  * round (a cooperative launch; where the device refuses it, latency mode runs instead).
- * Calls through hb_gate_batch* and the single-gate calls take it; captured graphs, the
- * no-allocation call, mixed launches and the serving loop keep latency mode. On by
- * default; hb_set_pair_mode(0) turns it off. Same output words as latency mode. */
+ * Calls through hb_gate_batch* and the single-gate calls take it, and so do the serving
+ * loop's small launches while its other launch slot is idle; captured graphs, the
+ * no-allocation call and hb_mixed_batch keep latency mode. On by default;
+ * hb_set_pair_mode(0) turns it off. Same output words as latency mode. */
 int hb_set_pair_mode(int enable);
 int hb_pair_mode(int* enabled);
 int hb_set_latency_mode(uint32_t max_gates);
@@ -349,6 +350,8 @@ typedef struct {
   double gate_p50_ms, gate_p99_ms;
   /* launches run in the throughput shape; largest launch so far (gates) */
   uint64_t batched_launches, max_launch_gates;
+  /* launches run in pair mode (each gate on two compute units) */
+  uint64_t pair_launches;
 } hb_server_stats_t;
 
 /* max_gates_per_launch: gate slots per launch. A launch takes at most one gate per compute

@@ -267,3 +267,25 @@ def test_pair_mode_zama_matches_latency_mode(hb, orc, keyset, count):
     sample = np.unique(np.r_[0:1, count - 1:count])
     ref = orc.OracleKeys(2, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=2)
     assert np.array_equal(paired[sample], ref)
+
+
+@pytest.mark.parametrize("count", [1, 33])
+def test_pair_mode_babybear_matches_latency_mode(hb, orc, keyset, count):
+    """BabyBear in pair mode (one digit row per wave 0-3 of each workgroup): the latency-mode
+    words and the oracle's (config 1's single NAND runs this path)."""
+    P, sk, bsk, ksk = keyset(0)
+    say(f"pair mode babybear, {count} gates")
+    if 0 not in _EVALUATORS:
+        _EVALUATORS[0] = hb.Evaluator(keys=(bsk, ksk), params=P)
+    ev = _EVALUATORS[0]
+    b0, b1, c0, c1 = _inputs_for(hb, sk, 0, 82, count)
+    paired = ev.gate_batch("nand", c0, c1)
+    hb.set_pair_mode(False)
+    try:
+        split = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_pair_mode(True)
+    assert np.array_equal(paired, split)
+    sample = np.unique(np.r_[0:1, count - 1:count])
+    ref = orc.OracleKeys(0, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=2)
+    assert np.array_equal(paired[sample], ref)

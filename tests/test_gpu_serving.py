@@ -234,3 +234,42 @@ def test_throughput_mode_equals_latency_mode(hb, keyset, clients):
         assert got == list(circ.eval_plain(bits)), k
     assert np.array_equal(out_t, a_out)
     assert np.array_equal(core_t, a_core)
+
+
+def test_small_launches_pair_mode_equals_latency_mode(hb, keyset, clients):
+    """Small launches (at most one gate per two CUs, one slot busy at a time) run in pair
+    mode over BabyBear and Goldilocks circuits together; every circuit's outputs are the same
+    words as with pair mode off (latency mode) and decrypt to the plaintext evaluation."""
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    P0, sk0, bsk0, ksk0 = keyset(hb.SET_BABYBEAR)
+    bb = (hb.Evaluator(keys=(bsk0, ksk0), params=P0), hb.Encryptor(sk0), hb.Decryptor(sk0))
+    pool = [bb, clients[0], bb, clients[1]]
+    rng = stream(58, "pair-serve")
+    jobs = []
+    for ev, enc, dec in pool:
+        circ = random_nand_circuit(rng, 4, 12)
+        bits = rng.integers(0, 2, size=12)
+        jobs.append((ev, dec, circ, bits, enc.encrypt_many(bits)))
+
+    def run():
+        srv = Server(max_gates_per_launch=64)
+        ids = [srv.submit(ev, circ, cts) for ev, dec, circ, bits, cts in jobs]
+        outs = [srv.wait(r)[0] for r in ids]
+        st = srv.stats()
+        srv.close()
+        return outs, st
+
+    assert hb.pair_mode()
+    paired, st = run()
+    say(f"pair stats {st}")
+    assert st["pair_launches"] >= 1
+    hb.set_pair_mode(False)
+    try:
+        plain, st_off = run()
+    finally:
+        hb.set_pair_mode(True)
+    assert st_off["pair_launches"] == 0
+    for k, ((ev, dec, circ, bits, cts), a, m) in enumerate(zip(jobs, plain, paired)):
+        assert np.array_equal(a, m), k
+        got = [dec.decrypt(hb.LWECiphertext(m[j])) for j in range(m.shape[0])]
+        assert got == list(circ.eval_plain(bits)), k
