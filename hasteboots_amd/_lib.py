@@ -58,6 +58,7 @@ class HbTuneResult(C.Structure):
 
 TUNE_VARIANTS = [(1, 1), (2, 1), (4, 1), (2, 2), (4, 2), (4, 3)]
 LATENCY_DEFAULT = 0xFFFFFFFF  # HB_LATENCY_DEFAULT
+PAIR_WAIT_DEFAULT = 0xFFFFFFFF  # HB_PAIR_WAIT_DEFAULT
 
 
 class HbError(RuntimeError):
@@ -136,6 +137,8 @@ _PROTOS = {
     "hb_set_latency_mode_for": (C.c_int, [C.c_int, C.c_uint32]),
     "hb_set_pair_mode": (C.c_int, [C.c_int]),
     "hb_pair_mode": (C.c_int, [C.POINTER(C.c_int)]),
+    "hb_set_pair_wait_limit": (C.c_int, [C.c_uint32]),
+    "hb_ek_pair_check": (C.c_int, [C.c_void_p]),
     "hb_latency_mode": (C.c_int, [C.POINTER(C.c_uint32)]),
     "hb_latency_mode_for": (C.c_int, [C.c_int, C.POINTER(C.c_uint32)]),
     "hb_modmul_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.POINTER(C.c_double),

@@ -454,6 +454,10 @@ bool pair_batch(const Engine& E, const ParamSet& p, size_t count) {
 }
 
 int ensure_pair_flags(hb_ek* ek, size_t count) {
+  if (!ek->pair_status) {
+    HIP_TRY(hipMalloc((void**)&ek->pair_status, sizeof(uint64_t)));
+    HIP_TRY(hipMemset(ek->pair_status, 0, sizeof(uint64_t)));
+  }
   if (count <= ek->pair_cap) return HB_OK;
   if (ek->acc_ev_used) HIP_TRY(hipEventSynchronize(ek->acc_ev));
   if (ek->pair_flags) HIP_TRY(hipFree(ek->pair_flags));
@@ -486,6 +490,8 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   if (pair) {
     a.pair_flags = ek->pair_flags;
     a.pair_epoch = ++ek->pair_epoch;
+    a.pair_polls = E.pair_polls;
+    a.pair_status = ek->pair_status;
     done = launch_bootstrap_pair(a, s) == hipSuccess;
     if (!done) (void)hipGetLastError();  // refused (co-residency): latency mode instead
   }
@@ -493,6 +499,22 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   rc = timer_stop(s);
   if (rc) return rc;
   return acc_release(ek, s);
+}
+
+// Pair-mode status of ek's launches since the last check, read on stream s once the
+// launches are complete: HB_ERR_DEVICE if a wait of one of them timed out (its outputs are
+// then wrong).
+int pair_check(hb_ek* ek, hipStream_t s) {
+  if (!ek->pair_status || ek->pair_checked == ek->pair_epoch) return HB_OK;
+  uint64_t v = 0;
+  HIP_TRY(hipMemcpyAsync(&v, ek->pair_status, sizeof v, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint32_t seen = ek->pair_checked;
+  ek->pair_checked = ek->pair_epoch;
+  if ((uint32_t)(v >> 32) > seen)
+    return fail(HB_ERR_DEVICE, "pair mode: an exchange wait of launch %u timed out; the outputs "
+                "of that launch are invalid", (unsigned)(v >> 32));
+  return HB_OK;
 }
 
 int ensure_mux(hb_ek* ek, size_t count) {
@@ -700,6 +722,7 @@ int hb_ek_destroy(hb_ek_t* ek) {
   if (ek->acc_dev) (void)hipFree(ek->acc_dev);
   if (ek->mux_dev) (void)hipFree(ek->mux_dev);
   if (ek->pair_flags) (void)hipFree(ek->pair_flags);
+  if (ek->pair_status) (void)hipFree(ek->pair_status);
   if (ek->acc_ev) (void)hipEventDestroy(ek->acc_ev);
   delete ek;
   return HB_OK;
@@ -772,6 +795,8 @@ int hb_gate_batch(hb_ek_t* ek, int gate, size_t count, const uint64_t* c0s, cons
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(outs, dout, bytes, hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  rc = pair_check(ek, E.stream);
+  if (rc) return rc;
   return record_ms(E.stream);
 }
 
@@ -1009,6 +1034,8 @@ int hb_bootstrap(hb_ek_t* ek, const uint64_t* c, const uint64_t* lut, uint64_t* 
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(out, dout, n1 * 8, hipMemcpyDeviceToHost, E.stream));
   HIP_TRY(hipStreamSynchronize(E.stream));
+  rc = pair_check(ek, E.stream);
+  if (rc) return rc;
   return record_ms(E.stream);
 }
 
@@ -1351,6 +1378,19 @@ int hb_set_pair_mode(int enable) {
   if (rc) return rc;
   engine().pair_mode = enable != 0;
   return HB_OK;
+}
+
+int hb_set_pair_wait_limit(uint32_t polls) {
+  int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
+  if (rc) return rc;
+  engine().pair_polls = polls == HB_PAIR_WAIT_DEFAULT ? kPairPollsDefault : polls;
+  return HB_OK;
+}
+
+int hb_ek_pair_check(hb_ek_t* ek) {
+  if (!ek) return fail(HB_ERR_ARG, "null ek");
+  std::lock_guard<std::mutex> g(ek->mu);
+  return pair_check(ek, engine().stream);
 }
 
 int hb_pair_mode(int* enabled) {

@@ -73,9 +73,11 @@ struct Engine {
   // 103.8, 2048 112.6 / 104.1; Zama 1536 138.5 / 159.0, 1792 161.7 / 160.4; BabyBear 1792
   // 27.8 / 28.5, 2048 31.5 / 28.5.
   uint32_t split_max[3] = {1792, 1792, 1536};
-  // Pair mode (Goldilocks, untraced batches within the latency-mode cutoff and at most one
-  // gate per two CUs): hb_set_pair_mode switches it.
+  // Pair mode (untraced batches within the latency-mode cutoff and at most one gate per
+  // two CUs): hb_set_pair_mode switches it; hb_set_pair_wait_limit sets the polls of a
+  // flag before a wait times out.
   bool pair_mode = true;
+  uint32_t pair_polls = kPairPollsDefault;
 };
 
 // Built-in latency-mode cutoff of a parameter set on a device with `simd_slots` SIMDs.
@@ -210,10 +212,13 @@ struct hb_ek {
   uint64_t* mux_dev = nullptr;
   size_t mux_cap = 0;  // gates
   // Pair mode's flag words (kPairFlagWords per gate, zeroed once) and the epoch of its last
-  // launch, guarded by acc_ev like acc_dev.
+  // launch, guarded by acc_ev like acc_dev; the status word (epoch << 32 of the last launch
+  // whose wait timed out) and the last epoch whose status the host has read.
   uint64_t* pair_flags = nullptr;
   size_t pair_cap = 0;  // gates
   uint32_t pair_epoch = 0;
+  uint64_t* pair_status = nullptr;
+  uint32_t pair_checked = 0;
 };
 
 namespace hb {

@@ -872,21 +872,22 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 // stores to complete (s_waitcnt vmcnt(0)) and stores the flag; the receiver polls the flag
 // and then loads the quarter, both past its L2. Agent-scope release/acquire fences (L2
 // write-back and invalidation every round) cost 6 ms at 64 gates.
-// Waits poll with relaxed agent-scope loads and end after kPairPolls polls; a timeout
+// Waits poll with relaxed agent-scope loads and end after A.pair_polls polls; a timeout
 // records the launch's epoch in flag word 10, which ends the waits of both workgroups,
-// and a wave whose wait failed waits no more (the outputs are then wrong), so every wave
+// and in the launch's status word, which the host reads after the launch and reports as
+// an error; a wave whose wait failed waits no more (its outputs are wrong), so every wave
 // reaches the end after at most one timeout.
 // ----------------------------------------------------------------------------
-constexpr u32 kPairPolls = 1u << 18;
-
-__device__ __forceinline__ bool pair_wait(const u64* f, u64 want, u64* abort_word, u64 epoch) {
-  for (u32 it = 0; it < kPairPolls; it++) {
+__device__ __forceinline__ bool pair_wait(const u64* f, u64 want, u64* abort_word, u64 epoch,
+                                          const BootArgs& A) {
+  for (u32 it = 0; it < A.pair_polls; it++) {
     if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
     if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
       return false;
     __builtin_amdgcn_s_sleep(1);
   }
   __hip_atomic_store(abort_word, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(A.pair_status, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return false;
 }
 
@@ -1015,7 +1016,7 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
       for (int k = 0; k < 4; k++) pair_put(to + k * 64 + lane, x[k]);
       pair_signal(fl + c * 4 + qw, want);
     } else {
-      alive = alive && pair_wait(fl + (1 - c) * 4 + qw, want, fl + 10, ep);
+      alive = alive && pair_wait(fl + (1 - c) * 4 + qw, want, fl + 10, ep, A);
       if (alive) {
         const W* from = xch + ((size_t)((1 - c) * 2 + par) * 4 + qw) * 256;
 #pragma unroll
@@ -1045,7 +1046,7 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
       pair_signal(fl + 9, ep | 1);
     } else {
       W b0 = 0;
-      if (alive && pair_wait(fl + 9, ep | 1, fl + 10, ep)) b0 = (W)pair_get(fl + 8);
+      if (alive && pair_wait(fl + 9, ep | 1, fl + 10, ep, A)) b0 = (W)pair_get(fl + 8);
       part[2 * N + lane] = b0;
       wave_sync();
       keyswitch_extract<F, PF, true>(A, g, part + N, part + 2 * N, lds, twl, itwl, lane);
@@ -1470,7 +1471,7 @@ static hipError_t launch_bootstrap_f(const BootArgs& a, int gpw, int wps, hipStr
 hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s) {
   const int db = dig_bits(a);
   const bool bb = !a.is_goldilocks && a.br_len <= 4;  // one digit row per wave 0-3
-  if ((db != 8 && db != 5 && !bb) || a.trace || a.idx || !a.pair_flags || a.count == 0)
+  if ((db != 8 && db != 5 && !bb) || a.trace || a.idx || !a.pair_flags || !a.pair_status || a.count == 0)
     return hipErrorInvalidValue;
   BootArgs arg = a;
   void* params[] = {&arg};

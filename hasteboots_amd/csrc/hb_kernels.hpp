@@ -48,12 +48,17 @@ struct BootArgs {
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
   uint64_t* pair_flags;  // pair mode: count * kPairFlagWords words, zeroed once
   uint32_t pair_epoch;   // pair mode: this launch's epoch (> every earlier one on pair_flags)
+  uint32_t pair_polls;   // pair mode: polls of a flag before its wait times out
+  uint64_t* pair_status; // pair mode: a launch whose wait timed out stores epoch << 32 here
 };
 
 // Pair mode's per-gate flag words (launch_bootstrap_pair): [0, 8) the quarter flags of the
 // two workgroups, 8 the b component's word 0 for the key switch, 9 its flag, 10 the epoch
 // of a launch whose wait timed out.
 constexpr int kPairFlagWords = 16;
+// Polls of a pair-mode flag before the wait times out (about 10 ms at one poll per
+// s_sleep 1; a round's exchange waits microseconds).
+constexpr uint32_t kPairPollsDefault = 1u << 18;
 
 // Per-gate scratch in BootArgs::acc, in words of the set's size: the accumulator RLWE (2N
 // words) then the rotated accumulator's byte-sized gadget digits, byte-transposed (one

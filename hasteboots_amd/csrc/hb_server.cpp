@@ -110,7 +110,9 @@ struct Slot {
   BootArgs* parts_host = nullptr;  // pinned
   void* acc = nullptr;
   uint64_t* pair_flags = nullptr;  // pair mode: kPairFlagWords per gate, zeroed once
+  uint64_t* pair_status = nullptr; // epoch << 32 of a pair launch whose wait timed out
   uint32_t pair_epoch = 0;
+  bool pair = false;  // the launch in flight runs in pair mode
   bool busy = false;
   std::vector<std::pair<Request*, uint32_t>> work;
   uint32_t gates = 0;
@@ -194,6 +196,21 @@ int finish(hb_server* S, Request* r) {
 }
 
 int complete(hb_server* S, Slot& sl) {
+  if (sl.pair) {
+    // a pair-mode wait that timed out left this launch's outputs wrong: stop with an error
+    // before any request takes them
+    uint64_t v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, sl.pair_status, sizeof v, hipMemcpyDeviceToHost, sl.s));
+    HIP_TRY(hipStreamSynchronize(sl.s));
+    sl.pair = false;
+    if ((uint32_t)(v >> 32) == sl.pair_epoch) {
+      // the slot is retired without finishing its requests: their waiters see the error
+      sl.work.clear();
+      sl.gates = 0;
+      sl.busy = false;
+      return fail(HB_ERR_DEVICE, "pair mode: an exchange wait of a serving launch timed out");
+    }
+  }
   float ms = 0;
   if (hipEventElapsedTime(&ms, sl.ev0, sl.ev1) == hipSuccess) S->st.busy_ms += ms;
   S->st.gates_done += sl.gates;
@@ -288,8 +305,11 @@ int launch(hb_server* S, Slot& sl) {
     for (uint32_t k = 0; k < nparts; k++) {
       sl.parts_host[k].pair_flags = sl.pair_flags + (size_t)sl.parts_host[k].first_block * kPairFlagWords;
       sl.parts_host[k].pair_epoch = sl.pair_epoch;
+      sl.parts_host[k].pair_polls = engine().pair_polls;
+      sl.parts_host[k].pair_status = sl.pair_status;
     }
   }
+  sl.pair = kind == MixedKernel::pair;
   HIP_TRY(hipMemcpyAsync(sl.parts_dev, sl.parts_host, nparts * sizeof(BootArgs),
                          hipMemcpyHostToDevice, sl.s));
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
@@ -410,6 +430,7 @@ void release_slot(Slot& sl) {
   if (sl.parts_host) (void)hipHostFree(sl.parts_host);
   if (sl.acc) (void)hipFree(sl.acc);
   if (sl.pair_flags) (void)hipFree(sl.pair_flags);
+  if (sl.pair_status) (void)hipFree(sl.pair_status);
   if (sl.ev0) (void)hipEventDestroy(sl.ev0);
   if (sl.ev1) (void)hipEventDestroy(sl.ev1);
   if (sl.s) (void)hipStreamDestroy(sl.s);
@@ -430,6 +451,8 @@ int server_alloc(hb_server* S, uint64_t hbm_budget, uint32_t max_gates) {
     const size_t pair_bytes = (size_t)max_gates * kPairFlagWords * sizeof(uint64_t);
     HIP_TRY(hipMalloc((void**)&sl.pair_flags, pair_bytes));
     HIP_TRY(hipMemset(sl.pair_flags, 0, pair_bytes));  // epochs start at 1
+    HIP_TRY(hipMalloc((void**)&sl.pair_status, sizeof(uint64_t)));
+    HIP_TRY(hipMemset(sl.pair_status, 0, sizeof(uint64_t)));
   }
   HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
   return HB_OK;

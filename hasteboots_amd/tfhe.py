@@ -19,7 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import LATENCY_DEFAULT, TUNE_VARIANTS, HbParams, HbTuneResult, check, lib, ptr
+from ._lib import LATENCY_DEFAULT, PAIR_WAIT_DEFAULT, TUNE_VARIANTS, HbParams, HbTuneResult, check, lib, ptr
 
 SET_BABYBEAR, SET_GOLDILOCKS, SET_ZAMA = 0, 1, 2
 GATES = {"nand": 0, "and": 1, "or": 2, "nor": 3, "xor": 4, "xnor": 5, "majority": 6,
@@ -288,6 +288,11 @@ class Evaluator:
         check(lib().hb_ek_export(self._h, ptr(bsk), ptr(ksk)))
         return bsk, ksk
 
+    def pair_check(self) -> None:
+        """Raises if a pair-mode wait of this key's launches since the last check timed out
+        (hb_ek_pair_check); call once the streams of device-buffer calls are synchronised."""
+        check(lib().hb_ek_pair_check(self._h))
+
     def _gate(self, gate: str, c0, c1, c2=None) -> LWECiphertext:
         out = np.zeros(self.params.n + 1, dtype=np.uint64)
         check(lib().hb_gate(self._h, GATES[gate], ptr(c0.words), ptr(c1.words),
@@ -546,7 +551,7 @@ def restore_latency_modes(modes) -> None:
 
 
 def pair_mode() -> bool:
-    """Whether small Goldilocks batches run one gate on two compute units (hb_pair_mode)."""
+    """Whether small batches run one gate on two compute units (hb_pair_mode)."""
     v = C.c_int()
     check(lib().hb_pair_mode(C.byref(v)))
     return bool(v.value)
@@ -554,6 +559,12 @@ def pair_mode() -> bool:
 
 def set_pair_mode(enable: bool) -> None:
     check(lib().hb_set_pair_mode(1 if enable else 0))
+
+
+def set_pair_wait_limit(polls: int = PAIR_WAIT_DEFAULT) -> None:
+    """Polls of a pair-mode flag before the wait times out (hb_set_pair_wait_limit); a
+    timed-out launch is reported as an error. PAIR_WAIT_DEFAULT restores the default."""
+    check(lib().hb_set_pair_wait_limit(polls))
 
 
 def mixed_batch(parts) -> list[np.ndarray]:

@@ -228,9 +228,20 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * Calls through hb_gate_batch* and the single-gate calls take it, and so do the serving
  * loop's small launches while its other launch slot is idle; captured graphs, the
  * no-allocation call and hb_mixed_batch keep latency mode. On by default;
- * hb_set_pair_mode(0) turns it off. Same output words as latency mode. */
+ * hb_set_pair_mode(0) turns it off. Same output words as latency mode.
+ * The two workgroups of a gate wait for each other's flags with a bounded number of polls
+ * (HB_PAIR_WAIT_DEFAULT: 2^18, about 10 ms, where a round's wait takes microseconds); a wait
+ * that runs out ends the launch's waits and leaves that launch's outputs wrong, and the
+ * launch is then reported as HB_ERR_DEVICE: by hb_gate_batch and the single-gate calls
+ * that ran it or that follow it on the same key, by hb_ek_pair_check (for
+ * hb_gate_batch_dev launches, once the caller has synchronised their streams), and by the
+ * serving loop, which stops with that error. hb_set_pair_wait_limit sets the polls per wait
+ * (tests force the timeout with 0). */
+#define HB_PAIR_WAIT_DEFAULT 0xFFFFFFFFu
 int hb_set_pair_mode(int enable);
 int hb_pair_mode(int* enabled);
+int hb_set_pair_wait_limit(uint32_t polls);
+int hb_ek_pair_check(hb_ek_t* ek);
 int hb_set_latency_mode(uint32_t max_gates);
 int hb_set_latency_mode_for(int set, uint32_t max_gates);
 int hb_latency_mode(uint32_t* max_gates);

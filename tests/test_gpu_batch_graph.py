@@ -289,3 +289,33 @@ def test_pair_mode_babybear_matches_latency_mode(hb, orc, keyset, count):
     sample = np.unique(np.r_[0:1, count - 1:count])
     ref = orc.OracleKeys(0, bsk, ksk).gate_batch(0, c0[sample], c1[sample], threads=2)
     assert np.array_equal(paired[sample], ref)
+
+
+def test_pair_wait_timeout_is_an_error(hb, keyset):
+    """A pair-mode wait that runs out is reported as an error by the call that ran it.
+    Forced from the host: with 0 polls per wait every wait of the launch fails at once (every
+    wave still reaches the end). With the default limit restored the same call gives the
+    latency-mode words, and a device-buffer launch is checked through pair_check."""
+    from hasteboots_amd._lib import HbError
+    P, sk, bsk, ksk = keyset(1)
+    say("pair wait limit 0: expecting an error")
+    if 1 not in _EVALUATORS:
+        _EVALUATORS[1] = hb.Evaluator(keys=(bsk, ksk), params=P)
+    ev = _EVALUATORS[1]
+    b0, b1, c0, c1 = _inputs_for(hb, sk, 1, 83, 4)
+    assert hb.pair_mode()
+    hb.set_pair_wait_limit(0)
+    try:
+        with pytest.raises(HbError) as ei:
+            ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_pair_wait_limit()
+    assert "timed out" in str(ei.value)
+    good = ev.gate_batch("nand", c0, c1)
+    ev.pair_check()
+    hb.set_pair_mode(False)
+    try:
+        split = ev.gate_batch("nand", c0, c1)
+    finally:
+        hb.set_pair_mode(True)
+    assert np.array_equal(good, split)

@@ -273,3 +273,33 @@ def test_small_launches_pair_mode_equals_latency_mode(hb, keyset, clients):
         assert np.array_equal(a, m), k
         got = [dec.decrypt(hb.LWECiphertext(m[j])) for j in range(m.shape[0])]
         assert got == list(circ.eval_plain(bits)), k
+
+
+def test_pair_wait_timeout_stops_the_server_with_an_error(hb, clients):
+    """A serving launch in pair mode whose waits run out (forced from the host with 0 polls
+    per wait) makes the request's wait raise instead of returning the launch's wrong words;
+    a new server with the default limit runs the same circuit to the plaintext result."""
+    from hasteboots_amd._lib import HbError
+    from hasteboots_amd.serving import Server, random_nand_circuit
+    ev, enc, dec = clients[0]
+    rng = stream(59, "pair-timeout")
+    circ = random_nand_circuit(rng, 2, 6)
+    bits = rng.integers(0, 2, size=6)
+    cts = enc.encrypt_many(bits)
+    hb.set_pair_wait_limit(0)
+    try:
+        srv = Server(max_gates_per_launch=64)
+        rid = srv.submit(ev, circ, cts)
+        with pytest.raises(HbError) as ei:
+            srv.wait(rid)
+        srv.close()
+    finally:
+        hb.set_pair_wait_limit()
+    assert "timed out" in str(ei.value)
+    srv = Server(max_gates_per_launch=64)
+    out, _ = srv.wait(srv.submit(ev, circ, cts))
+    st = srv.stats()
+    srv.close()
+    assert st["pair_launches"] >= 1
+    got = [dec.decrypt(hb.LWECiphertext(out[k])) for k in range(out.shape[0])]
+    assert got == list(circ.eval_plain(bits))
