@@ -212,6 +212,9 @@ int hb::ensure_engine(int device) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
       E.simd_slots = (uint32_t)prop.multiProcessorCount * 4;
     for (int set = 0; set < SET_COUNT; set++) E.split_max[set] = default_split_max(set, E.simd_slots);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+      E.wall_khz = (uint32_t)khz;
   }
   {
     void* p = nullptr;
@@ -455,8 +458,9 @@ bool pair_batch(const Engine& E, const ParamSet& p, size_t count) {
 
 int ensure_pair_flags(hb_ek* ek, size_t count) {
   if (!ek->pair_status) {
-    HIP_TRY(hipMalloc((void**)&ek->pair_status, sizeof(uint64_t)));
-    HIP_TRY(hipMemset(ek->pair_status, 0, sizeof(uint64_t)));
+    HIP_TRY(hipMalloc((void**)&ek->pair_status, kPairStatusWords * sizeof(uint64_t)));
+    HIP_TRY(hipMemset(ek->pair_status, 0, kPairStatusWords * sizeof(uint64_t)));
+    ek->pair_tickets = 0;
   }
   if (count <= ek->pair_cap) return HB_OK;
   if (ek->acc_ev_used) HIP_TRY(hipEventSynchronize(ek->acc_ev));
@@ -490,10 +494,12 @@ int launch_gates(hb_ek* ek, int gate, size_t count, const uint64_t* c0, const ui
   if (pair) {
     a.pair_flags = ek->pair_flags;
     a.pair_epoch = ++ek->pair_epoch;
-    a.pair_polls = E.pair_polls;
     a.pair_status = ek->pair_status;
-    done = launch_bootstrap_pair(a, s) == hipSuccess;
-    if (!done) (void)hipGetLastError();  // refused (co-residency): latency mode instead
+    a.pair_ticket0 = ek->pair_tickets;
+    a.pair_wait_ticks = E.pair_wait_ticks();
+    HIP_TRY(launch_bootstrap_pair(a, s));
+    ek->pair_tickets += 2 * (uint64_t)count;
+    done = true;
   }
   if (!done) HIP_TRY(launch_shaped(E, p, a, s));
   rc = timer_stop(s);
@@ -1380,10 +1386,10 @@ int hb_set_pair_mode(int enable) {
   return HB_OK;
 }
 
-int hb_set_pair_wait_limit(uint32_t polls) {
+int hb_set_pair_wait_limit(uint32_t microseconds) {
   int rc = ensure_engine(engine().device < 0 ? 0 : engine().device);
   if (rc) return rc;
-  engine().pair_polls = polls == HB_PAIR_WAIT_DEFAULT ? kPairPollsDefault : polls;
+  engine().pair_wait_us = microseconds == HB_PAIR_WAIT_DEFAULT ? kPairWaitDefaultUs : microseconds;
   return HB_OK;
 }
 

@@ -74,10 +74,12 @@ struct Engine {
   // 27.8 / 28.5, 2048 31.5 / 28.5.
   uint32_t split_max[3] = {1792, 1792, 1536};
   // Pair mode (untraced batches within the latency-mode cutoff and at most one gate per
-  // two CUs): hb_set_pair_mode switches it; hb_set_pair_wait_limit sets the polls of a
-  // flag before a wait times out.
+  // two CUs): hb_set_pair_mode switches it; hb_set_pair_wait_limit sets how long a wait
+  // lasts before it times out. wall_khz: the rate of the device clock wall_clock64 reads.
   bool pair_mode = true;
-  uint32_t pair_polls = kPairPollsDefault;
+  uint32_t pair_wait_us = kPairWaitDefaultUs;
+  uint32_t wall_khz = 100000;
+  uint64_t pair_wait_ticks() const { return (uint64_t)pair_wait_us * wall_khz / 1000; }
 };
 
 // Built-in latency-mode cutoff of a parameter set on a device with `simd_slots` SIMDs.
@@ -173,12 +175,8 @@ inline hipError_t launch_mixed_kernel(MixedKernel k, const BootArgs* parts_dev, 
     case MixedKernel::split:
       return launch_mixed_split(parts_dev, nparts, blocks, mixed_split_waves(narrow), traced, s);
     case MixedKernel::one_wave: return launch_mixed(parts_dev, nparts, blocks, s);
-    case MixedKernel::pair: {
-      const int cls = mixed_class(sets);  // 1, 2, 3 one set, 0 a mix
-      if (launch_mixed_pair(parts_dev, nparts, blocks, cls, s) == hipSuccess) return hipSuccess;
-      (void)hipGetLastError();  // refused (co-residency): latency mode, same block numbering
-      return launch_mixed_split(parts_dev, nparts, blocks, 8, false, s);
-    }
+    case MixedKernel::pair:
+      return launch_mixed_pair(parts_dev, nparts, blocks, mixed_class(sets), s);
     default: return launch_mixed_batch(parts_dev, nparts, blocks, mixed_class(sets), s);
   }
 }
@@ -217,8 +215,9 @@ struct hb_ek {
   uint64_t* pair_flags = nullptr;
   size_t pair_cap = 0;  // gates
   uint32_t pair_epoch = 0;
-  uint64_t* pair_status = nullptr;
+  uint64_t* pair_status = nullptr;  // kPairStatusWords: timeout epoch, ticket counter
   uint32_t pair_checked = 0;
+  uint64_t pair_tickets = 0;  // workgroups of the pair launches issued so far
 };
 
 namespace hb {

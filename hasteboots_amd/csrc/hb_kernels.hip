@@ -854,8 +854,11 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 }
 
 // ----------------------------------------------------------------------------
-// Pair mode (Goldilocks with byte-digit gadgets, untraced): one gate on two workgroups of 8
-// wavefronts, blocks 2g (c = 0) and 2g + 1 (c = 1), on two CUs of one cooperative launch.
+// Pair mode (untraced; Goldilocks with byte-digit gadgets, BabyBear with at most 4 digits):
+// one gate on two workgroups of 8 wavefronts on two CUs. Workgroups pair up in the order
+// they start: each takes a ticket from the launch's counter (pair_ticket), and tickets 2g
+// (c = 0) and 2g + 1 (c = 1) carry gate g, so a workgroup only ever waits for one that is
+// running or for the next one of its launch to start, whatever else holds the CUs.
 // Workgroup c owns accumulator component c: it rotates it ([C | M] in LDS, as
 // bootstrap_gate_split keeps it), wave w runs digit row w of it (DIG = 8: one digit per
 // wave, half of a latency-mode wave's rows; DIG = 5: one or two of the 13), and after the
@@ -872,23 +875,47 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
 // stores to complete (s_waitcnt vmcnt(0)) and stores the flag; the receiver polls the flag
 // and then loads the quarter, both past its L2. Agent-scope release/acquire fences (L2
 // write-back and invalidation every round) cost 6 ms at 64 gates.
-// Waits poll with relaxed agent-scope loads and end after A.pair_polls polls; a timeout
-// records the launch's epoch in flag word 10, which ends the waits of both workgroups,
-// and in the launch's status word, which the host reads after the launch and reports as
-// an error; a wave whose wait failed waits no more (its outputs are wrong), so every wave
-// reaches the end after at most one timeout.
+// Waits poll with relaxed agent-scope loads and end after A.pair_wait_ticks ticks of the
+// device's constant-rate clock (0: at once, without polling); a timeout records the
+// launch's epoch in flag word 10, which ends the waits of both workgroups, and in the
+// launch's status word, which the host reads after the launch and reports as an error; a
+// wave whose wait failed waits no more (its outputs are wrong), so every wave reaches the
+// end after at most one timeout.
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ bool pair_wait(const u64* f, u64 want, u64* abort_word, u64 epoch,
                                           const BootArgs& A) {
-  for (u32 it = 0; it < A.pair_polls; it++) {
-    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
-    if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
-      return false;
-    __builtin_amdgcn_s_sleep(1);
+  if (A.pair_wait_ticks != 0) {
+    const u64 t0 = wall_clock64();
+    for (;;) {
+      if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want) return true;
+      if (__hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch)
+        return false;
+      if (wall_clock64() - t0 > A.pair_wait_ticks) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
   }
   __hip_atomic_store(abort_word, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(A.pair_status, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return false;
+}
+
+// This workgroup's ticket in its launch: the launch's workgroups take consecutive values of
+// the counter pair_status[1] from pair_ticket0 on (launches on one counter run one after
+// another, and each of the 2 * count workgroups takes one). A ticket outside [0, tickets)
+// would mean a counter out of step with the host: the workgroup records the launch as
+// failed and touches nothing else (its partner's wait then times out).
+__device__ __forceinline__ bool pair_ticket(u64* status, u64 ticket0, u32 tickets, u32 epoch,
+                                            u32* slot, u32* t) {
+  if (threadIdx.x == 0) {
+    const u64 v = __hip_atomic_fetch_add(status + 1, (u64)1, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT) - ticket0;
+    *slot = v < tickets ? (u32)v : ~0u;
+    if (v >= tickets)
+      __hip_atomic_store(status, (u64)epoch << 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  *t = *slot;
+  return *t != ~0u;
 }
 
 __device__ __forceinline__ void pair_put(u64* p, u64 v) {
@@ -1109,7 +1136,8 @@ __global__ __launch_bounds__(64 * WAVES) void bootstrap_split_kernel(BootArgs A)
                                        part, aa, ab, twl, itwl);
 }
 
-// Pair-mode launch: workgroups 2g and 2g + 1 carry gate g (bootstrap_gate_pair).
+// Pair-mode launch of 2 * count workgroups: tickets 2g and 2g + 1 carry gate g
+// (bootstrap_gate_pair).
 template <class F, int DIG>
 __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
   using W = typename DF<F>::W;
@@ -1118,13 +1146,17 @@ __global__ __launch_bounds__(64 * 8) void bootstrap_pair_kernel(BootArgs A) {
   __shared__ W twl[kTwxWords];
   __shared__ W itwl[1024];
   __shared__ W part[4 * 2 * 1024];
+  __shared__ u32 tk;
+  u32 t;
+  if (!pair_ticket(A.pair_status, A.pair_ticket0, 2 * A.count, A.pair_epoch, &tk, &t)) return;
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * 8);
-  bootstrap_gate_pair<F, DIG>(A, (int)(blockIdx.x >> 1), (int)(blockIdx.x & 1),
-                              lds[threadIdx.x >> 6], 1024 + 64, ms, part, twl, itwl);
+  bootstrap_gate_pair<F, DIG>(A, (int)(t >> 1), (int)(t & 1), lds[threadIdx.x >> 6], 1024 + 64,
+                              ms, part, twl, itwl);
 }
 
-// Pair-mode mixed launch (the serving loop): workgroups 2b and 2b + 1 carry gate b of the
-// launch, which belongs to the last part whose first_block <= b (every part BabyBear with
+// Pair-mode mixed launch (the serving loop): tickets 2b and 2b + 1 (from the counter and
+// base of part 0, which every part shares) carry gate b of the launch, which belongs to the
+// last part whose first_block <= b (every part BabyBear with
 // at most 4 digits or Goldilocks with a byte-digit gadget, untraced, with its own
 // pair_flags rows). CLS: 1 every part Goldilocks with 8-bit digits, 2 every part with
 // 5-bit digits, 3 every part BabyBear, 0 any of them (runtime branch).
@@ -1136,11 +1168,17 @@ __global__ __launch_bounds__(64 * 8) void mixed_pair_kernel(const BootArgs* __re
   __shared__ u64 twl[kTwxWords];
   __shared__ u64 itwl[1024];
   __shared__ u64 part[4 * 2 * 1024];
-  const u32 gb = blockIdx.x >> 1;
+  __shared__ u32 tk;
+  const BootArgs& L = parts[nparts - 1];  // the launch's gates end with the last part's
+  u32 t;
+  if (!pair_ticket(parts[0].pair_status, parts[0].pair_ticket0, 2 * (L.first_block + L.count),
+                   parts[0].pair_epoch, &tk, &t))
+    return;
+  const u32 gb = t >> 1;
   int p = 0;
   while (p + 1 < nparts && gb >= parts[p + 1].first_block) p++;
   const BootArgs A = parts[p];
-  const int g = (int)(gb - A.first_block), c = (int)(blockIdx.x & 1), w = threadIdx.x >> 6;
+  const int g = (int)(gb - A.first_block), c = (int)(t & 1), w = threadIdx.x >> 6;
   if (CLS == 3 || (CLS == 0 && !A.is_goldilocks)) {
     if constexpr (CLS == 0 || CLS == 3) {
       // BabyBear: the same LDS arrays in 32-bit words
@@ -1473,12 +1511,15 @@ hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s) {
   const bool bb = !a.is_goldilocks && a.br_len <= 4;  // one digit row per wave 0-3
   if ((db != 8 && db != 5 && !bb) || a.trace || a.idx || !a.pair_flags || !a.pair_status || a.count == 0)
     return hipErrorInvalidValue;
-  BootArgs arg = a;
-  void* params[] = {&arg};
-  const void* f = bb        ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<Bb, 0>)
-                  : db == 8 ? reinterpret_cast<const void*>(&bootstrap_pair_kernel<Gl, 8>)
-                            : reinterpret_cast<const void*>(&bootstrap_pair_kernel<Gl, 5>);
-  return hipLaunchCooperativeKernel(f, dim3(2 * a.count), dim3(64 * 8), params, 0, s);
+  const BootArgs& arg = a;
+  const dim3 grid(2 * a.count), block(64 * 8);
+  if (bb)
+    hipLaunchKernelGGL((bootstrap_pair_kernel<Bb, 0>), grid, block, 0, s, arg);
+  else if (db == 8)
+    hipLaunchKernelGGL((bootstrap_pair_kernel<Gl, 8>), grid, block, 0, s, arg);
+  else
+    hipLaunchKernelGGL((bootstrap_pair_kernel<Gl, 5>), grid, block, 0, s, arg);
+  return hipGetLastError();
 }
 
 template <bool TR>
@@ -1543,12 +1584,14 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, u32 total_g
 hipError_t launch_mixed_pair(const BootArgs* parts_dev, int nparts, u32 total_gates, int cls,
                              hipStream_t s) {
   if (total_gates == 0) return hipErrorInvalidValue;
-  void* params[] = {&parts_dev, &nparts};
-  const void* f = cls == 1   ? reinterpret_cast<const void*>(&mixed_pair_kernel<1>)
-                  : cls == 2 ? reinterpret_cast<const void*>(&mixed_pair_kernel<2>)
-                  : cls == 3 ? reinterpret_cast<const void*>(&mixed_pair_kernel<3>)
-                             : reinterpret_cast<const void*>(&mixed_pair_kernel<0>);
-  return hipLaunchCooperativeKernel(f, dim3(2 * total_gates), dim3(64 * 8), params, 0, s);
+  const dim3 grid(2 * total_gates), block(64 * 8);
+  switch (cls) {
+    case 1: hipLaunchKernelGGL(mixed_pair_kernel<1>, grid, block, 0, s, parts_dev, nparts); break;
+    case 2: hipLaunchKernelGGL(mixed_pair_kernel<2>, grid, block, 0, s, parts_dev, nparts); break;
+    case 3: hipLaunchKernelGGL(mixed_pair_kernel<3>, grid, block, 0, s, parts_dev, nparts); break;
+    default: hipLaunchKernelGGL(mixed_pair_kernel<0>, grid, block, 0, s, parts_dev, nparts);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, u32 total_gates, hipStream_t s) {

@@ -48,17 +48,21 @@ struct BootArgs {
   void* trace;           // optional: traced (vfhe) semantics, count*(n+1)*2N acc history
   uint64_t* pair_flags;  // pair mode: count * kPairFlagWords words, zeroed once
   uint32_t pair_epoch;   // pair mode: this launch's epoch (> every earlier one on pair_flags)
-  uint32_t pair_polls;   // pair mode: polls of a flag before its wait times out
-  uint64_t* pair_status; // pair mode: a launch whose wait timed out stores epoch << 32 here
+  uint64_t* pair_status;  // pair mode: [0] epoch << 32 of a launch whose wait timed out,
+                          // [1] the workgroup ticket counter (pair_ticket)
+  uint64_t pair_ticket0;  // pair mode: the counter's value when this launch starts
+  uint64_t pair_wait_ticks;  // pair mode: wall-clock ticks before a wait times out (0: at once)
 };
 
 // Pair mode's per-gate flag words (launch_bootstrap_pair): [0, 8) the quarter flags of the
 // two workgroups, 8 the b component's word 0 for the key switch, 9 its flag, 10 the epoch
 // of a launch whose wait timed out.
 constexpr int kPairFlagWords = 16;
-// Polls of a pair-mode flag before the wait times out (about 10 ms at one poll per
-// s_sleep 1; a round's exchange waits microseconds).
-constexpr uint32_t kPairPollsDefault = 1u << 18;
+// Pair-mode wait limit, microseconds: a round's exchange waits microseconds, a workgroup
+// waiting for its launch's next workgroup to start waits for a CU to free up.
+constexpr uint32_t kPairWaitDefaultUs = 2000000;
+// Device words of a pair-mode status (pair_status): timeout epoch, ticket counter.
+constexpr int kPairStatusWords = 2;
 
 // Per-gate scratch in BootArgs::acc, in words of the set's size: the accumulator RLWE (2N
 // words) then the rotated accumulator's byte-sized gadget digits, byte-transposed (one
@@ -69,9 +73,11 @@ constexpr size_t kScratchWordsPerGate = 4 * 1024;
 // targeted by register allocation; (gpw, wps) in {(1,1), (2,1), (4,1), (2,2), (4,2), (4,3)}.
 hipError_t launch_bootstrap(const BootArgs& a, int is_goldilocks, int gpw, int wps,
                             hipStream_t s);
-// Pair mode (Goldilocks with 8-bit digits, untraced): one gate on two workgroups of 8
-// waves, on two CUs, one accumulator component each, as one cooperative launch of 2 * count
-// workgroups; the caller falls back to latency mode when the launch is refused.
+// Pair mode (untraced; Goldilocks with a byte-digit gadget or BabyBear with at most 4
+// digits): one gate on two workgroups of 8 waves, on two CUs, one accumulator component
+// each, 2 * count workgroups paired in the order they start (pair_flags, pair_epoch,
+// pair_status, pair_ticket0 and pair_wait_ticks set; launches on one pair_status run one
+// after another).
 hipError_t launch_bootstrap_pair(const BootArgs& a, hipStream_t s);
 // Latency mode: `waves` (4 or 8) wavefronts per gate (traced gates: 8 only).
 hipError_t launch_bootstrap_split(const BootArgs& a, int is_goldilocks, int waves,
@@ -83,9 +89,10 @@ hipError_t launch_mixed_split(const BootArgs* parts_dev, int nparts, uint32_t to
                               int waves, bool traced, hipStream_t s);
 hipError_t launch_mixed(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                         hipStream_t s);
-// Pair-mode mixed launch (cooperative; the parts' first_block count gates, and every part
-// is Goldilocks with a byte-digit gadget, untraced, with pair_flags and pair_epoch set):
-// cls 1 every part 8-bit digits, 2 every part 5-bit, 0 either.
+// Pair-mode mixed launch (the parts' first_block count gates; every part untraced,
+// Goldilocks with a byte-digit gadget or BabyBear with at most 4 digits, with the pair
+// fields set as above and one pair_status): cls 1 every part Goldilocks with 8-bit digits,
+// 2 every part 5-bit, 3 every part BabyBear, 0 any.
 hipError_t launch_mixed_pair(const BootArgs* parts_dev, int nparts, uint32_t total_gates,
                              int cls, hipStream_t s);
 // Throughput-mode mixed launch: kMixedBatchGpw gates of one part per workgroup, two waves

@@ -110,7 +110,8 @@ struct Slot {
   BootArgs* parts_host = nullptr;  // pinned
   void* acc = nullptr;
   uint64_t* pair_flags = nullptr;  // pair mode: kPairFlagWords per gate, zeroed once
-  uint64_t* pair_status = nullptr; // epoch << 32 of a pair launch whose wait timed out
+  uint64_t* pair_status = nullptr; // kPairStatusWords: timeout epoch, ticket counter
+  uint64_t pair_tickets = 0;       // workgroups of the pair launches issued so far
   uint32_t pair_epoch = 0;
   bool pair = false;  // the launch in flight runs in pair mode
   bool busy = false;
@@ -293,8 +294,10 @@ int launch(hb_server* S, Slot& sl) {
     narrow = narrow && split8_fits(a.br_len, a.br_bits);
   }
   for (const auto& w : sl.work) sets |= 1 << w.first->ek->p.set;
-  // pair mode only while the other slot has no kernel in flight: a pair launch needs all
-  // its workgroups resident at once
+  // pair mode only while the other slot has no kernel in flight: beside another launch's
+  // workgroups (whose LDS leaves no room for a pair workgroup on their CUs) the pair
+  // workgroups would start as CUs free up, correct (they pair up as they start) but slower
+  // than latency mode
   bool other_busy = false;
   for (const Slot& o : S->slots) other_busy = other_busy || (&o != &sl && o.busy);
   const MixedKernel kind = mixed_kernel_for(engine(), sets, used, any_traced, narrow,
@@ -305,8 +308,9 @@ int launch(hb_server* S, Slot& sl) {
     for (uint32_t k = 0; k < nparts; k++) {
       sl.parts_host[k].pair_flags = sl.pair_flags + (size_t)sl.parts_host[k].first_block * kPairFlagWords;
       sl.parts_host[k].pair_epoch = sl.pair_epoch;
-      sl.parts_host[k].pair_polls = engine().pair_polls;
       sl.parts_host[k].pair_status = sl.pair_status;
+      sl.parts_host[k].pair_ticket0 = sl.pair_tickets;
+      sl.parts_host[k].pair_wait_ticks = engine().pair_wait_ticks();
     }
   }
   sl.pair = kind == MixedKernel::pair;
@@ -315,6 +319,7 @@ int launch(hb_server* S, Slot& sl) {
   HIP_TRY(hipEventRecord(sl.ev0, sl.s));
   HIP_TRY(launch_mixed_kernel(kind, sl.parts_dev, (int)nparts, blocks, narrow, sets, sl.s,
                               any_traced));
+  if (kind == MixedKernel::pair) sl.pair_tickets += 2 * (uint64_t)used;
   HIP_TRY(hipEventRecord(sl.ev1, sl.s));
   sl.busy = true;
   sl.gates = used;
@@ -451,8 +456,8 @@ int server_alloc(hb_server* S, uint64_t hbm_budget, uint32_t max_gates) {
     const size_t pair_bytes = (size_t)max_gates * kPairFlagWords * sizeof(uint64_t);
     HIP_TRY(hipMalloc((void**)&sl.pair_flags, pair_bytes));
     HIP_TRY(hipMemset(sl.pair_flags, 0, pair_bytes));  // epochs start at 1
-    HIP_TRY(hipMalloc((void**)&sl.pair_status, sizeof(uint64_t)));
-    HIP_TRY(hipMemset(sl.pair_status, 0, sizeof(uint64_t)));
+    HIP_TRY(hipMalloc((void**)&sl.pair_status, kPairStatusWords * sizeof(uint64_t)));
+    HIP_TRY(hipMemset(sl.pair_status, 0, kPairStatusWords * sizeof(uint64_t)));
   }
   HIP_TRY(hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking));
   return HB_OK;

@@ -561,10 +561,11 @@ def set_pair_mode(enable: bool) -> None:
     check(lib().hb_set_pair_mode(1 if enable else 0))
 
 
-def set_pair_wait_limit(polls: int = PAIR_WAIT_DEFAULT) -> None:
-    """Polls of a pair-mode flag before the wait times out (hb_set_pair_wait_limit); a
-    timed-out launch is reported as an error. PAIR_WAIT_DEFAULT restores the default."""
-    check(lib().hb_set_pair_wait_limit(polls))
+def set_pair_wait_limit(microseconds: int = PAIR_WAIT_DEFAULT) -> None:
+    """How long a pair-mode wait lasts before it times out (hb_set_pair_wait_limit); a
+    timed-out launch is reported as an error. PAIR_WAIT_DEFAULT restores the default (2 s),
+    0 makes every wait time out at once."""
+    check(lib().hb_set_pair_wait_limit(microseconds))
 
 
 def mixed_batch(parts) -> list[np.ndarray]:

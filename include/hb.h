@@ -224,23 +224,25 @@ int hb_set_tuned_config(int set, int gates_per_workgroup, int waves_per_simd);
  * the latency-mode cutoff and of at most one gate per two compute units (128 on MI355X) run
  * each gate on two workgroups of 8 wavefronts on two compute units, one accumulator
  * component each, exchanging half of the external product through global memory every
- * round (a cooperative launch; where the device refuses it, latency mode runs instead).
+ * round. Workgroups pair up in the order they start, so a launch needs no compute units
+ * reserved for it and runs beside kernels on other streams.
  * Calls through hb_gate_batch* and the single-gate calls take it, and so do the serving
  * loop's small launches while its other launch slot is idle; captured graphs, the
  * no-allocation call and hb_mixed_batch keep latency mode. On by default;
  * hb_set_pair_mode(0) turns it off. Same output words as latency mode.
- * The two workgroups of a gate wait for each other's flags with a bounded number of polls
- * (HB_PAIR_WAIT_DEFAULT: 2^18, about 10 ms, where a round's wait takes microseconds); a wait
- * that runs out ends the launch's waits and leaves that launch's outputs wrong, and the
- * launch is then reported as HB_ERR_DEVICE: by hb_gate_batch and the single-gate calls
- * that ran it or that follow it on the same key, by hb_ek_pair_check (for
- * hb_gate_batch_dev launches, once the caller has synchronised their streams), and by the
- * serving loop, which stops with that error. hb_set_pair_wait_limit sets the polls per wait
- * (tests force the timeout with 0). */
+ * The two workgroups of a gate wait for each other's flags for a bounded time
+ * (HB_PAIR_WAIT_DEFAULT: 2 s; a round's wait takes microseconds, a workgroup waiting for its
+ * partner to start waits for a compute unit to free up); a wait that runs out ends the
+ * launch's waits and leaves that launch's outputs wrong, and the launch is then reported as
+ * HB_ERR_DEVICE: by hb_gate_batch and the single-gate calls that ran it or that follow it
+ * on the same key, by hb_ek_pair_check (for hb_gate_batch_dev launches, once the caller
+ * has synchronised their streams), and by the serving loop, which stops with that error.
+ * hb_set_pair_wait_limit sets the limit in microseconds (0: every wait times out at once,
+ * which tests use to force the error). */
 #define HB_PAIR_WAIT_DEFAULT 0xFFFFFFFFu
 int hb_set_pair_mode(int enable);
 int hb_pair_mode(int* enabled);
-int hb_set_pair_wait_limit(uint32_t polls);
+int hb_set_pair_wait_limit(uint32_t microseconds);
 int hb_ek_pair_check(hb_ek_t* ek);
 int hb_set_latency_mode(uint32_t max_gates);
 int hb_set_latency_mode_for(int set, uint32_t max_gates);

@@ -319,3 +319,52 @@ def test_pair_wait_timeout_is_an_error(hb, keyset):
     finally:
         hb.set_pair_mode(True)
     assert np.array_equal(good, split)
+
+
+def test_pair_launch_beside_another_keys_launch_on_another_stream(hb, keyset):
+    """A pair-mode launch (8 gates of one key) issued on one stream while a 512-gate
+    latency-mode launch of another key runs on a second stream, with no host synchronisation
+    between them: the pair workgroups start as compute units free up and pair in the order
+    they start, so both launches give the words they give alone and no wait times out."""
+    import ctypes as C
+    from hasteboots_amd._lib import check, lib
+    L = lib()
+    P, sk_a, bsk_a, ksk_a = keyset(1)
+    _, sk_b, bsk_b, ksk_b = keyset(1, 1)
+    ev_a = hb.Evaluator(keys=(bsk_a, ksk_a), params=P)
+    ev_b = hb.Evaluator(keys=(bsk_b, ksk_b), params=P)
+    say("pair launch beside another stream's latency-mode launch")
+    _, _, a0, a1 = _inputs_for(hb, sk_a, 1, 84, 512)
+    _, _, p0, p1 = _inputs_for(hb, sk_b, 1, 85, 8)
+    expect_a = ev_a.gate_batch("nand", a0, a1)
+    expect_b = ev_b.gate_batch("nand", p0, p1)
+
+    def dev(x):
+        d = C.c_void_p()
+        check(L.hb_dev_alloc(x.nbytes, C.byref(d)))
+        check(L.hb_memcpy_h2d(d, x.ctypes.data, x.nbytes))
+        return d
+
+    streams = [C.c_void_p(), C.c_void_p()]
+    for s in streams:
+        check(L.hb_stream_create(C.byref(s)))
+    bufs = [(dev(a0), dev(a1), dev(np.zeros_like(a0))), (dev(p0), dev(p1), dev(np.zeros_like(p0)))]
+    try:
+        for rep in range(2):
+            (d0, d1, do), (e0, e1, eo) = bufs
+            check(L.hb_gate_batch_dev(ev_a._h, 0, 512, d0, d1, None, do, streams[0]))
+            check(L.hb_gate_batch_dev(ev_b._h, 0, 8, e0, e1, None, eo, streams[1]))
+            for s in streams:
+                check(L.hb_stream_sync(s))
+            ev_b.pair_check()
+            for (_, _, o), e in zip(bufs, (expect_a, expect_b)):
+                got = np.zeros_like(e)
+                check(L.hb_memcpy_d2h(got.ctypes.data, o, got.nbytes))
+                assert np.array_equal(got, e), rep
+    finally:
+        for trio in bufs:
+            for d in trio:
+                L.hb_dev_free(d)
+        for s in streams:
+            L.hb_stream_destroy(s)
+        del ev_a, ev_b
