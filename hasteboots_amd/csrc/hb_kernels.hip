@@ -1640,7 +1640,10 @@ static void launch_ntt_f(int inverse, u64* data, size_t count, const void* tw, u
 // Launch shapes measured on MI355X (4096 Goldilocks polynomials, kernel us, forward /
 // inverse): N = 1024 one polynomial per wave, 4 waves per workgroup: 22.5 / 25.4 (8 persistent
 // waves per workgroup with the next polynomial prefetched: 23.2 / 26.2, since at two waves
-// per SIMD the VALU, not the loads, sets the pace); N = 2048 with half-size exchange buffers,
+// per SIMD the VALU, not the loads, sets the pace; two 4-wave workgroups per CU, each wave
+// two polynomials, the next one copied into a per-wave LDS stage by global_load_lds_dwordx4
+// during the current transform, no registers held: 25.6 / 29.0 against 24.2 / 27.6 in the
+// same run); N = 2048 with half-size exchange buffers,
 // two 4-wave workgroups per CU: 45.0 / 53.6 (full buffers, one per CU: 62 / 73).
 hipError_t launch_ntt(int is_goldilocks, int inverse, u32 log_n, u64* data, size_t count,
                       const void* tw, u64 ninv, u32 cus, hipStream_t s) {

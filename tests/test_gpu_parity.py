@@ -77,6 +77,28 @@ def test_ntt_matches_oracle(hlib, orc, gold, q, log_n):
     check(hlib.hb_dev_free(d))
 
 
+@pytest.mark.parametrize("gold,q", [(True, GP), (False, BP)], ids=["goldilocks", "babybear"])
+@pytest.mark.parametrize("log_n", [10, 11])
+@pytest.mark.parametrize("count", [2049, 4099])
+def test_ntt_batches_beyond_resident_waves(hlib, orc, gold, q, log_n, count):
+    """Batches larger than the resident waves (two 4-wave workgroups per CU): waves take
+    a second and third polynomial, which N = 1024 stages through LDS while transforming
+    the one before; every polynomial equals the oracle's transform and round-trips."""
+    from hasteboots_amd._lib import check
+    say(f"ntt batch gold={gold} log_n={log_n} count={count}")
+    n = 1 << log_n
+    x = uniform_field(stream(23, f"ntt-batch{log_n}-{count}"), q, (count, n))
+    d = dev_buffer(hlib, x)
+    check(hlib.hb_ntt_dev(int(gold), 0, log_n, d, count, None))
+    check(hlib.hb_sync())
+    y = dev_read(hlib, d, x)
+    assert np.array_equal(y, orc.ntt(gold, x))
+    check(hlib.hb_ntt_dev(int(gold), 1, log_n, d, count, None))
+    check(hlib.hb_sync())
+    assert np.array_equal(dev_read(hlib, d, x), x)
+    check(hlib.hb_dev_free(d))
+
+
 @pytest.mark.parametrize("gold,q,bits,length", [(True, GP, 5, 13), (True, GP, 8, 8),
                                                 (False, BP, 8, 4), (False, BP, 5, 7)])
 def test_external_product_matches_oracle(hlib, orc, gold, q, bits, length):
