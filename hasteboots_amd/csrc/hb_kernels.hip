@@ -947,7 +947,11 @@ __device__ __forceinline__ void bootstrap_gate_pair(const BootArgs& A, int g, in
   constexpr int E = 16;
   constexpr int N = 64 * E;
   constexpr bool PF = true;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index as a scalar (wave_id) for BabyBear and Zama: one NAND 4.80 -> 4.69 and
+  // 16.2 -> 15.8 ms; Goldilocks with byte digits 9.6 -> 9.8 ms with it, so per lane there
+  constexpr bool kScalarWave = !(F::IS_GOLDILOCKS && DIG == 8);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = kScalarWave ? wave_id() : tid >> 6;
   const int ch = wave >> 2, qw = wave & 3;  // the output half and quarter this wave finishes
   const bool fin = ch == c;                 // ... in this workgroup; the others send theirs
   const u32 n = A.n;
@@ -1111,7 +1115,10 @@ void bootstrap_kernel(BootArgs A) {
   __shared__ W twl[kTwxWords];
   __shared__ W itwl[1024];
   stage_twiddles<W>(A.tw_fwd, A.tw_inv, twl, itwl, 1024, threadIdx.x, 64 * GPW);
-  const int w = threadIdx.x >> 6;
+  // The wave index as a scalar (wave_id) for Zama's 5-bit digits only: 4096 gates at (4, 2)
+  // 310.6 -> 305.9 ms, while Goldilocks with byte digits (201.9 -> 212.9 ms) and BabyBear
+  // (56.8 -> 69.9 ms) ran slower with it (same box, its spill 39 -> 2 VGPRs notwithstanding).
+  const int w = DIG == 5 ? wave_id() : (int)(threadIdx.x >> 6);
   const u32 g = blockIdx.x * GPW + w;
   if (g >= A.count) return;  // after the only workgroup barrier
   // Twiddles a stage ahead with one wave per SIMD, and for BabyBear, whose 32-bit words

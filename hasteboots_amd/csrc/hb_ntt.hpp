@@ -25,6 +25,14 @@
 
 namespace hb {
 
+// The wavefront's index in its one-dimensional workgroup as a wave-uniform (scalar) value.
+// threadIdx.x >> 6 is the same in the 64 lanes of a wave, but the compiler takes it for a
+// per-lane value, and every address derived from it (a gate's scratch, accumulator and
+// outputs) would then sit in VGPR pairs.
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
 // Intra-wave LDS ordering: all lanes of one wavefront execute LDS operations in
 // order; the fences stop the compiler from moving accesses across the exchange.
 __device__ __forceinline__ void wave_sync() {
