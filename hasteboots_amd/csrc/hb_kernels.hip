@@ -684,7 +684,10 @@ __device__ __forceinline__ void bootstrap_gate_split(const BootArgs& A, int g,
   // in step, so they hide little of each other's LDS latency (256 Goldilocks gates with 8
   // waves: 18.8 -> 17.9 ms).
   constexpr bool PF = true;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wave index as a scalar (wave_id) for BabyBear: 1024 gates 16.5 -> 16.2 ms; Goldilocks
+  // (57.0 -> 59.3 ms) and Zama (92.7 -> 96.6 ms) slower with it, so per lane there
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = F::IS_GOLDILOCKS ? tid >> 6 : wave_id();
   const int comp = wave / Q, q = wave % Q;
   const u32 n = A.n;
   const W* __restrict__ bsk = static_cast<const W*>(A.bsk);
