@@ -646,8 +646,8 @@ __device__ __forceinline__ void bootstrap_gate(const BootArgs& A, int g, typenam
     // NTTRLWE::inverse_transform_inplace then RLWE::add_assign_element_wise
     // (rlwe.rs:765-777, 287-290): raw-word add of the canonical product.
     // groups of 16 words for Goldilocks with byte digits, 4 otherwise (4096 gates at (4, 2),
-    // word by word / 4 / 16: Goldilocks 200.6 / 196.0 / 195.1 ms; Zama 304.1 / 297.7 ms
-    // (8: 297.7); BabyBear 57.0 / 53.3 ms (8: 53.4))
+    // word by word / 4 / 16: Goldilocks 200.6 / 196.0 / 195.1 ms; Zama 304.1 / 297.7 / 298.2 ms
+    // (8: 297.7); BabyBear 57.0 / 53.3 / 53.3 ms (8: 53.4))
     constexpr int kAccGroup = F::IS_GOLDILOCKS && DIG == 8 ? 16 : 4;
     ntt_inv_wave<F, E, PF>(pa, lds, itwl, lane);
     acc_add<F, E, kAccGroup>(outA, accA, pa, lane);
